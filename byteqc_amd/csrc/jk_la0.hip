@@ -1,0 +1,5 @@
+// Instantiates the J/K kernels whose bra shell has angular momentum 0.
+#include "jk_launch.cuh"
+namespace mjk {
+template hipError_t launch_la<0>(int, int, int, bool, const JKArgs &, hipStream_t, long *);
+}

@@ -1,0 +1,413 @@
+// Host side of libmi355jk.so: the C ABI declared in include/mi355jk.h.
+//
+// Ownership: every device buffer a cache needs is allocated here and freed by
+// mjk_del_basis_prod; input host tables are copied, nothing is borrowed.  Errors are
+// reported by return code plus a thread-local message; nothing here exits the process.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/mi355jk.h"
+#include "mjk_internal.h"
+
+using namespace mjk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *where)
+{
+    return fail(MJK_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(call)                                           \
+    do {                                                        \
+        hipError_t e_ = (call);                                 \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);       \
+    } while (0)
+
+constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8;
+constexpr int ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2, NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6;
+constexpr int PTR_COORD = 1;
+
+double fac_sp_host(int l) { return l == 0 ? 0.282094791773878143 : (l == 1 ? 0.488602511902919921 : 1.0); }
+
+// A list of device allocations released together.
+struct DevPool {
+    std::vector<void *> ptrs;
+    template <typename T>
+    hipError_t alloc(T **p, size_t n)
+    {
+        *p = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+    template <typename T>
+    hipError_t upload(T **p, const T *h, size_t n)
+    {
+        hipError_t e = alloc(p, n);
+        if (e != hipSuccess || n == 0) return e;
+        return hipMemcpy(*p, h, n * sizeof(T), hipMemcpyHostToDevice);
+    }
+    void release()
+    {
+        for (void *p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct ClassType {
+    int l_bra, l_ket, nprim_bra, nprim_ket, nprim12, npairs;
+    int pair_off, prim_off;
+};
+
+}  // namespace
+
+struct mjk_basis_prod {
+    int device = 0;
+    int nbas = 0, nao = 0, ncptype = 0, n_pairs = 0, n_prim_pairs = 0;
+    std::vector<ClassType> cp;
+    DevPool pool;
+    double *bx = nullptr, *by = nullptr, *bz = nullptr;
+    int *ao_loc = nullptr;
+    int *pair_i = nullptr, *pair_j = nullptr;
+    double *pair_q = nullptr;
+    double *a12 = nullptr, *e12 = nullptr, *x12 = nullptr, *y12 = nullptr, *z12 = nullptr;
+    double *dm_cond = nullptr;            // nbas * nbas workspace
+    unsigned long long *dm_max = nullptr;  // scalar (double bits)
+    unsigned long long *counters = nullptr;  // [0] fp64 quartets, [1] fp32 quartets
+    long last_launches = 0;
+};
+
+extern "C" {
+
+int mjk_version(void) { return MJK_VERSION; }
+
+const char *mjk_last_error(void) { return g_last_error.c_str(); }
+
+int mjk_nao(const mjk_basis_prod *bp) { return bp ? bp->nao : -1; }
+
+int mjk_q_cond(const int32_t *atm, int natm, const int32_t *bas, int nbas, const double *env,
+               double *q)
+{
+    if (!atm || !bas || !env || !q || nbas <= 0 || natm <= 0) return fail(MJK_ERR_ARG, "mjk_q_cond: bad argument");
+    std::vector<double> x(nbas), y(nbas), z(nbas), exps, coefs;
+    std::vector<int> nprim(nbas), poff(nbas), ls(nbas);
+    for (int i = 0; i < nbas; ++i) {
+        const int32_t *b = bas + i * BAS_SLOTS;
+        if (b[NCTR_OF] != 1) return fail(MJK_ERR_ARG, "mjk_q_cond: basis must be segmented (nctr == 1)");
+        if (b[ANG_OF] > MJK_LMAX) return fail(MJK_ERR_UNSUPPORTED, "mjk_q_cond: l > MJK_LMAX");
+        if (b[ATOM_OF] < 0 || b[ATOM_OF] >= natm) return fail(MJK_ERR_ARG, "mjk_q_cond: bad atom index");
+        const double *r = env + atm[b[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+        x[i] = r[0]; y[i] = r[1]; z[i] = r[2];
+        ls[i] = b[ANG_OF];
+        nprim[i] = b[NPRIM_OF];
+        poff[i] = (int)exps.size();
+        for (int p = 0; p < b[NPRIM_OF]; ++p) {
+            exps.push_back(env[b[PTR_EXP] + p]);
+            coefs.push_back(env[b[PTR_COEFF] + p]);
+        }
+    }
+    // pairs grouped by canonical (l_big, l_small)
+    std::vector<int> pi[4][4], pj[4][4];
+    for (int i = 0; i < nbas; ++i)
+        for (int j = 0; j <= i; ++j) {
+            int a = i, b = j;
+            if (ls[a] < ls[b]) std::swap(a, b);
+            pi[ls[a]][ls[b]].push_back(a);
+            pj[ls[a]][ls[b]].push_back(b);
+        }
+    DevPool pool;
+    ShellTab s{};
+    double *dx, *dy, *dz, *dexp, *dcoef, *dq;
+    int *dnp, *dpo;
+    hipError_t e;
+#define UP(ptr, vec) if ((e = pool.upload(&ptr, vec.data(), vec.size())) != hipSuccess) { pool.release(); return hip_fail(e, "mjk_q_cond upload"); }
+    UP(dx, x) UP(dy, y) UP(dz, z) UP(dexp, exps) UP(dcoef, coefs) UP(dnp, nprim) UP(dpo, poff)
+    if ((e = pool.alloc(&dq, (size_t)nbas * nbas)) != hipSuccess) { pool.release(); return hip_fail(e, "mjk_q_cond alloc"); }
+    s.x = dx; s.y = dy; s.z = dz; s.nprim = dnp; s.poff = dpo; s.exps = dexp; s.coefs = dcoef; s.nbas = nbas;
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b <= a; ++b) {
+            int n = (int)pi[a][b].size();
+            if (!n) continue;
+            int *dpi, *dpj;
+            UP(dpi, pi[a][b]) UP(dpj, pj[a][b])
+            if ((e = launch_qcond(a, b, s, dpi, dpj, n, dq, 0)) != hipSuccess) { pool.release(); return hip_fail(e, "mjk_q_cond launch"); }
+        }
+#undef UP
+    e = hipMemcpy(q, dq, sizeof(double) * nbas * nbas, hipMemcpyDeviceToHost);
+    pool.release();
+    if (e != hipSuccess) return hip_fail(e, "mjk_q_cond copy");
+    return MJK_OK;
+}
+
+int mjk_init_basis_prod(mjk_basis_prod **pbp, double diag_fac, const int32_t *bas_pair2shls,
+                        const int32_t *bas_pairs_locs, int ncptype, const double *pair_q,
+                        const int32_t *atm, int natm, const int32_t *bas, int nbas,
+                        const double *env)
+{
+    if (!pbp || !bas_pair2shls || !bas_pairs_locs || !pair_q || !atm || !bas || !env || ncptype < 0 || nbas <= 0)
+        return fail(MJK_ERR_ARG, "mjk_init_basis_prod: bad argument");
+    *pbp = nullptr;
+    const int n_pairs = bas_pairs_locs[ncptype];
+    const int32_t *p2i = bas_pair2shls, *p2j = bas_pair2shls + n_pairs;
+    std::vector<int> ao_loc(nbas + 1, 0);
+    std::vector<double> x(nbas), y(nbas), z(nbas);
+    for (int i = 0; i < nbas; ++i) {
+        const int32_t *b = bas + i * BAS_SLOTS;
+        if (b[NCTR_OF] != 1) return fail(MJK_ERR_ARG, "mjk_init_basis_prod: basis must be segmented (nctr == 1)");
+        if (b[ANG_OF] > MJK_LMAX || b[ANG_OF] < 0)
+            return fail(MJK_ERR_UNSUPPORTED, "mjk_init_basis_prod: angular momentum above MJK_LMAX");
+        if (b[ATOM_OF] < 0 || b[ATOM_OF] >= natm) return fail(MJK_ERR_ARG, "mjk_init_basis_prod: bad atom index");
+        const double *r = env + atm[b[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+        x[i] = r[0]; y[i] = r[1]; z[i] = r[2];
+        const int l = b[ANG_OF];
+        ao_loc[i + 1] = ao_loc[i] + (l + 1) * (l + 2) / 2;
+    }
+    auto *bp = new (std::nothrow) mjk_basis_prod();
+    if (!bp) return fail(MJK_ERR_OOM, "mjk_init_basis_prod: host allocation failed");
+    (void)hipGetDevice(&bp->device);
+    bp->nbas = nbas;
+    bp->nao = ao_loc[nbas];
+    bp->ncptype = ncptype;
+    bp->n_pairs = n_pairs;
+    std::vector<double> a12, e12, x12, y12, z12;
+    std::vector<int> pi(n_pairs), pj(n_pairs);
+    int prim_total = 0;
+    for (int c = 0; c < ncptype; ++c) {
+        ClassType ct{};
+        ct.pair_off = bas_pairs_locs[c];
+        ct.npairs = bas_pairs_locs[c + 1] - bas_pairs_locs[c];
+        ct.prim_off = prim_total;
+        if (ct.npairs > 0) {
+            const int i0 = p2i[ct.pair_off], j0 = p2j[ct.pair_off];
+            ct.l_bra = bas[i0 * BAS_SLOTS + ANG_OF];
+            ct.l_ket = bas[j0 * BAS_SLOTS + ANG_OF];
+            ct.nprim_bra = bas[i0 * BAS_SLOTS + NPRIM_OF];
+            ct.nprim_ket = bas[j0 * BAS_SLOTS + NPRIM_OF];
+            if (ct.l_bra < ct.l_ket) {
+                delete bp;
+                return fail(MJK_ERR_ARG, "mjk_init_basis_prod: pair classes need l_i >= l_j");
+            }
+        }
+        ct.nprim12 = ct.nprim_bra * ct.nprim_ket;
+        for (int pr = ct.pair_off; pr < ct.pair_off + ct.npairs; ++pr) {
+            const int i = p2i[pr], j = p2j[pr];
+            if (i < 0 || i >= nbas || j < 0 || j >= nbas) {
+                delete bp;
+                return fail(MJK_ERR_ARG, "mjk_init_basis_prod: shell index out of range");
+            }
+            const int32_t *bi = bas + i * BAS_SLOTS, *bj = bas + j * BAS_SLOTS;
+            if (bi[ANG_OF] != ct.l_bra || bj[ANG_OF] != ct.l_ket || bi[NPRIM_OF] != ct.nprim_bra ||
+                bj[NPRIM_OF] != ct.nprim_ket) {
+                delete bp;
+                return fail(MJK_ERR_ARG, "mjk_init_basis_prod: pair class mixes (l, nprim) patterns");
+            }
+            if (pr > ct.pair_off && pair_q[pr] > pair_q[pr - 1]) {
+                delete bp;
+                return fail(MJK_ERR_ARG, "mjk_init_basis_prod: pair_q must be sorted descending in each class");
+            }
+            pi[pr] = i;
+            pj[pr] = j;
+            const double *ai = env + bi[PTR_EXP], *aj = env + bj[PTR_EXP];
+            const double *ci = env + bi[PTR_COEFF], *cj = env + bj[PTR_COEFF];
+            const double rx = x[i] - x[j], ry = y[i] - y[j], rz = z[i] - z[j];
+            const double rr = rx * rx + ry * ry + rz * rz;
+            const double norm = fac_sp_host(ct.l_bra) * fac_sp_host(ct.l_ket) * (i == j ? diag_fac : 1.0);
+            for (int ip = 0; ip < ct.nprim_bra; ++ip)
+                for (int jp = 0; jp < ct.nprim_ket; ++jp) {
+                    const double aij = ai[ip] + aj[jp];
+                    a12.push_back(aij);
+                    e12.push_back(norm * ci[ip] * cj[jp] * exp(-rr * ai[ip] * aj[jp] / aij));
+                    x12.push_back((ai[ip] * x[i] + aj[jp] * x[j]) / aij);
+                    y12.push_back((ai[ip] * y[i] + aj[jp] * y[j]) / aij);
+                    z12.push_back((ai[ip] * z[i] + aj[jp] * z[j]) / aij);
+                }
+        }
+        prim_total += ct.npairs * ct.nprim12;
+        bp->cp.push_back(ct);
+    }
+    bp->n_prim_pairs = prim_total;
+    hipError_t e;
+#define UP(ptr, vec) if ((e = bp->pool.upload(&bp->ptr, vec.data(), vec.size())) != hipSuccess) goto oom;
+    UP(bx, x) UP(by, y) UP(bz, z) UP(ao_loc, ao_loc) UP(pair_i, pi) UP(pair_j, pj)
+    UP(a12, a12) UP(e12, e12) UP(x12, x12) UP(y12, y12) UP(z12, z12)
+#undef UP
+    if ((e = bp->pool.upload(&bp->pair_q, pair_q, (size_t)n_pairs)) != hipSuccess) goto oom;
+    if ((e = bp->pool.alloc(&bp->dm_cond, (size_t)nbas * nbas)) != hipSuccess) goto oom;
+    if ((e = bp->pool.alloc(&bp->dm_max, 1)) != hipSuccess) goto oom;
+    if ((e = bp->pool.alloc(&bp->counters, 2)) != hipSuccess) goto oom;
+    if ((e = hipMemset(bp->counters, 0, 2 * sizeof(unsigned long long))) != hipSuccess) goto oom;
+    *pbp = bp;
+    return MJK_OK;
+oom:
+    bp->pool.release();
+    delete bp;
+    return fail(e == hipErrorOutOfMemory ? MJK_ERR_OOM : MJK_ERR_HIP,
+                std::string("mjk_init_basis_prod: ") + hipGetErrorString(e));
+}
+
+int mjk_del_basis_prod(mjk_basis_prod **pbp)
+{
+    if (!pbp || !*pbp) return MJK_OK;
+    (*pbp)->pool.release();
+    delete *pbp;
+    *pbp = nullptr;
+    return MJK_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+PairSet pair_set(const mjk_basis_prod *bp, const ClassType &ct)
+{
+    PairSet s{};
+    s.ish = bp->pair_i + ct.pair_off;
+    s.jsh = bp->pair_j + ct.pair_off;
+    s.q = bp->pair_q + ct.pair_off;
+    s.a = bp->a12 + ct.prim_off;
+    s.e = bp->e12 + ct.prim_off;
+    s.px = bp->x12 + ct.prim_off;
+    s.py = bp->y12 + ct.prim_off;
+    s.pz = bp->z12 + ct.prim_off;
+    s.npairs = ct.npairs;
+    s.nprim = ct.nprim12;
+    return s;
+}
+
+int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
+
+int run_class_pair(mjk_basis_prod *bp, double *vj, double *vk, const double *dm, int n_dm,
+                   int cp_ij, int cp_kl, const mjk_screen &sc, bool dens, hipStream_t st)
+{
+    const ClassType &c1 = bp->cp[cp_ij], &c2 = bp->cp[cp_kl];
+    if (c1.npairs == 0 || c2.npairs == 0) return MJK_OK;
+    const ClassType *bra = &c1, *ket = &c2;
+    if (ncart_h(c1.l_bra) * ncart_h(c1.l_ket) < ncart_h(c2.l_bra) * ncart_h(c2.l_ket)) std::swap(bra, ket);
+    JKArgs a{};
+    a.bra = pair_set(bp, *bra);
+    a.ket = pair_set(bp, *ket);
+    a.bx = bp->bx; a.by = bp->by; a.bz = bp->bz;
+    a.ao_loc = bp->ao_loc;
+    a.dm = dm; a.vj = vj; a.vk = vk;
+    a.dm_cond = dens ? bp->dm_cond : nullptr;
+    a.dm_max = dens ? (const double *)bp->dm_max : nullptr;
+    a.tol = sc.direct_scf_tol;
+    a.nbas = bp->nbas; a.nao = bp->nao; a.n_dm = n_dm;
+    a.same = cp_ij == cp_kl;
+    const double pc = sc.precision_cutoff;
+    long nt = 0;
+    // FP64 part: bound >= precision_cutoff (everything when the cutoff is 0)
+    a.lo = pc > 0 ? pc : 0.0;
+    a.hi = INFINITY;
+    a.counter = bp->counters;
+    hipError_t e = launch_class(bra->l_bra, bra->l_ket, ket->l_bra, ket->l_ket, false, a, st, &nt);
+    if (e != hipSuccess)
+        return fail(MJK_ERR_HIP, std::string("mjk_build_jk: launch of (") + "spdf"[bra->l_bra] + "spdf"[bra->l_ket] + "|" +
+                                 "spdf"[ket->l_bra] + "spdf"[ket->l_ket] + ") failed: " + hipGetErrorString(e));
+    if (pc > 0) {
+        a.lo = 0.0;
+        a.hi = pc;
+        a.counter = bp->counters + 1;
+        e = launch_class(bra->l_bra, bra->l_ket, ket->l_bra, ket->l_ket, true, a, st, &nt);
+        if (e != hipSuccess) return hip_fail(e, "mjk_build_jk: FP32 launch");
+    }
+    bp->last_launches += nt > 0 ? 1 + (pc > 0) : 0;
+    return MJK_OK;
+}
+
+int check_common(mjk_basis_prod *bp, const double *dm, int nao, int n_dm, const mjk_screen *screen)
+{
+    if (!bp) return fail(MJK_ERR_ARG, "null basis-product cache");
+    if (!dm) return fail(MJK_ERR_ARG, "null density");
+    if (!screen) return fail(MJK_ERR_ARG, "null screen options");
+    if (nao != bp->nao) return fail(MJK_ERR_ARG, "nao does not match the cache");
+    if (n_dm <= 0) return fail(MJK_ERR_ARG, "n_dm must be positive");
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != bp->device) return fail(MJK_ERR_ARG, "current device differs from the cache's device");
+    return MJK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mjk_build_jk(mjk_basis_prod *bp, double *vj, double *vk, const double *dm, int nao, int n_dm,
+                 int cp_ij, int cp_kl, const mjk_screen *screen, void *stream)
+{
+    int rc = check_common(bp, dm, nao, n_dm, screen);
+    if (rc) return rc;
+    if (cp_ij < 0 || cp_kl < 0 || cp_ij >= bp->ncptype || cp_kl >= bp->ncptype || cp_kl > cp_ij)
+        return fail(MJK_ERR_ARG, "mjk_build_jk: class indices must satisfy 0 <= cp_kl <= cp_ij < ncptype");
+    if (!vj && !vk) return MJK_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const bool dens = screen->density_screen != 0;
+    if (dens) {
+        hipError_t e = launch_dm_cond(dm, n_dm, nao, bp->ao_loc, bp->nbas, bp->dm_cond, bp->dm_max, st);
+        if (e != hipSuccess) return hip_fail(e, "mjk_build_jk: density screen");
+    }
+    return run_class_pair(bp, vj, vk, dm, n_dm, cp_ij, cp_kl, *screen, dens, st);
+}
+
+int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *dm, int nao, int n_dm,
+                     const mjk_screen *screen, void *stream)
+{
+    int rc = check_common(bp, dm, nao, n_dm, screen);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)n_dm * nao * nao;
+    hipError_t e;
+    if (vj && (e = launch_fill(vj, n, 0.0, st)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: zero vj");
+    if (vk && (e = launch_fill(vk, n, 0.0, st)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: zero vk");
+    if ((e = launch_fill_u64(bp->counters, 2, st)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: counters");
+    bp->last_launches = 0;
+    if (!vj && !vk) return MJK_OK;
+    const bool dens = screen->density_screen != 0;
+    if (dens && (e = launch_dm_cond(dm, n_dm, nao, bp->ao_loc, bp->nbas, bp->dm_cond, bp->dm_max, st)) != hipSuccess)
+        return hip_fail(e, "mjk_build_jk_all: density screen");
+    // heavy classes first
+    std::vector<std::pair<double, std::pair<int, int>>> order;
+    for (int i = 0; i < bp->ncptype; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const ClassType &a = bp->cp[i], &b = bp->cp[j];
+            const double w = (double)a.npairs * b.npairs * a.nprim12 * b.nprim12 *
+                             ncart_h(a.l_bra) * ncart_h(a.l_ket) * ncart_h(b.l_bra) * ncart_h(b.l_ket);
+            order.push_back({-w, {i, j}});
+        }
+    std::sort(order.begin(), order.end());
+    for (auto &o : order) {
+        rc = run_class_pair(bp, vj, vk, dm, n_dm, o.second.first, o.second.second, *screen, dens, st);
+        if (rc) return rc;
+    }
+    if (vj && (e = launch_symmetrize(vj, n_dm, nao, 2.0, st)) != hipSuccess) return hip_fail(e, "symmetrize vj");
+    if (vk && (e = launch_symmetrize(vk, n_dm, nao, 1.0, st)) != hipSuccess) return hip_fail(e, "symmetrize vk");
+    return MJK_OK;
+}
+
+int mjk_last_stats(mjk_basis_prod *bp, double *out)
+{
+    if (!bp || !out) return fail(MJK_ERR_ARG, "mjk_last_stats: bad argument");
+    unsigned long long c[2];
+    hipError_t e = hipMemcpy(c, bp->counters, sizeof(c), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "mjk_last_stats");
+    out[0] = (double)c[0];
+    out[1] = (double)c[1];
+    out[2] = (double)bp->last_launches;
+    return MJK_OK;
+}
+
+}  // extern "C"

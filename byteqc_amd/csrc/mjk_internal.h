@@ -1,0 +1,25 @@
+// Internal declarations shared by the host dispatcher and the helper kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "jk_device.cuh"
+
+namespace mjk {
+
+// Shells of a Cartesian, segmented basis on the device.
+struct ShellTab {
+    const double *x, *y, *z;     // centre of each shell
+    const int *nprim, *poff;     // primitive count and offset into exps / coefs
+    const double *exps, *coefs;  // coefficients carry the radial normalisation
+    int nbas;
+};
+
+hipError_t launch_qcond(int la, int lb, const ShellTab &s, const int *pi, const int *pj, int n,
+                        double *q, hipStream_t st);
+hipError_t launch_dm_cond(const double *dm, int n_dm, int nao, const int *ao_loc, int nbas,
+                          double *dc, unsigned long long *dmax, hipStream_t st);
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long *p, size_t n, hipStream_t st);
+hipError_t launch_symmetrize(double *v, int n_dm, int nao, double f, hipStream_t st);
+hipError_t launch_class(int la, int lb, int lc, int ld, bool fp32, const JKArgs &a, hipStream_t st, long *nt);
+
+}  // namespace mjk
