@@ -1,0 +1,221 @@
+#!/usr/bin/env python
+"""Benchmark: direct-SCF J/K build on one MI355X.
+
+Workload (default): 64 seeded water molecules in a 16 Angstrom box, def2-TZVP (2752
+spherical / 3072 Cartesian AOs), SAD density, Schwarz cutoff 1e-13 with density-weighted
+screening, plain FP64.  One step = one full J/K build (mjk_build_jk_all) with the
+density already in HBM; `value` is seconds per build (lower is better).
+
+Also reported:
+  roofline      the most expensive class kernel, timed per launch with HIP events on the
+                stream it runs on; achieved = its algorithmic FLOPs (byteqc_amd/perf/flops.py
+                x quartets it computed) / launch time, against the 78.6 TFLOP/s FP64 peak
+  cpu_baseline  the CPU oracle's Rys J/K (C, OpenMP, all host cores) on the same shells,
+                screening and density, timed on a systematic sample of bra-pair rows and
+                scaled to the full build
+  scf           (--scf) RHF time-to-1e-8 Eh from the SAD guess
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def per_class_profile(vhfopt, dmc, vj, vk, screen, torch, lib, mjk):
+    """Time every class pair separately (HIP events on the launch stream)."""
+    from byteqc_amd.perf.flops import class_flops, class_label
+    stream = torch.cuda.current_stream()
+    locs = vhfopt.bas_pairs_locs
+    bas = vhfopt.bas
+    p2s = vhfopt.bas_pair2shls
+    info = []
+    for c in range(vhfopt.ncptype):
+        if locs[c + 1] > locs[c]:
+            i, j = p2s[0, locs[c]], p2s[1, locs[c]]
+            info.append((int(bas[i, 1]), int(bas[j, 1]), int(bas[i, 2] * bas[j, 2])))
+        else:
+            info.append(None)
+    sc2 = mjk.Screen(screen.direct_scf_tol, screen.precision_cutoff, 2 if screen.density_screen else 0)
+    # block maxima for density_screen == 2
+    mjk.check(lib.mjk_build_jk_all(vhfopt.bp, mjk.ptr(vj), mjk.ptr(vk), mjk.ptr(dmc), vhfopt.nao, 1,
+                                   ctypes.byref(screen), ctypes.c_void_p(stream.cuda_stream)), 'warm')
+    torch.cuda.synchronize()
+    out = []
+    prev = np.zeros(3)
+    mjk.check(lib.mjk_last_stats(vhfopt.bp, mjk.ptr(prev)), 'stats')
+    for a in range(vhfopt.ncptype):
+        for b in range(a + 1):
+            if info[a] is None or info[b] is None:
+                continue
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            mjk.check(lib.mjk_build_jk(vhfopt.bp, mjk.ptr(vj), mjk.ptr(vk), mjk.ptr(dmc), vhfopt.nao, 1,
+                                       a, b, ctypes.byref(sc2), ctypes.c_void_p(stream.cuda_stream)),
+                      'mjk_build_jk')
+            e1.record(stream)
+            e1.synchronize()
+            cur = np.zeros(3)
+            mjk.check(lib.mjk_last_stats(vhfopt.bp, mjk.ptr(cur)), 'stats')
+            nq = cur[0] - prev[0] + cur[1] - prev[1]
+            prev = cur
+            la, lb, pa = info[a]
+            lc, ld, pb = info[b]
+            fl = nq * class_flops(la, lb, lc, ld, pa * pb)
+            out.append({'class': class_label(la, lb, lc, ld), 'cp': (a, b), 'nprim': pa * pb,
+                        'quartets': int(nq), 'ms': e0.elapsed_time(e1), 'flops': fl})
+    return out
+
+
+def cpu_baseline(vhfopt, dm_cart_host, target_s):
+    import oracle
+    sb = oracle.SegBasis(vhfopt.atm, vhfopt.bas, vhfopt.env, vhfopt.coeff)
+    nb = sb.nbas
+    npair = nb * (nb + 1) // 2
+    cores = os.cpu_count() or 1
+    stride = max(1, npair // 400)
+    times = []
+    t_all = 0.0
+    offset = 0
+    while t_all < target_s and offset < stride:
+        t0 = time.perf_counter()
+        oracle.rys_jk_cart(sb, dm_cart_host, cutoff=vhfopt.direct_scf_tol, nthreads=cores,
+                           q=vhfopt.q_cond, stride=stride, offset=offset)
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        t_all += dt
+        offset += max(1, stride // 7)
+    est = float(np.mean(times)) * stride
+    return {'value': est, 'unit': 's/iter', 'cores': cores, 'kind': 'port',
+            'sample': f'{len(times)} systematic samples of bra-pair rows (1 in {stride} of {npair} '
+                      f'shell pairs each, {t_all:.1f} s measured), scaled by {stride}'}
+
+
+def run_scf(mol, precision_cutoff=0.0):
+    from byteqc_amd.scf.rhf import RHF
+    mf = RHF(mol, conv_tol=1e-8, precision_cutoff=precision_cutoff)
+    mf.build()
+    e = mf.kernel()
+    return {'e_tot': e, 'converged': mf.converged, 'cycles': mf.cycles,
+            'time_s': mf.timings['total'], 'jk_s': mf.timings['jk'],
+            'precision_cutoff': precision_cutoff}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--workload', default='h2o64-def2tzvp')
+    ap.add_argument('--precision-cutoff', type=float, default=0.0)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-classes', action='store_true')
+    ap.add_argument('--scf', action='store_true')
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU only (got --gpus {args.gpus}); the J/K build has no '
+              f'multi-GPU path in this build', file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print('bench.py needs a GPU', file=sys.stderr)
+        return 2
+    from byteqc_amd import lib as mjk
+    from byteqc_amd.perf.workloads import make_mol, WORKLOADS
+    from byteqc_amd.scf import hf
+    from byteqc_amd.scf.rhf import init_guess_by_atom
+    lib = mjk.load()
+
+    mol = make_mol(args.workload)
+    log(f'workload {args.workload}: natm={mol.natm} nao={mol.nao}')
+    t0 = time.perf_counter()
+    vhfopt = hf._VHFOpt(mol).build(1e-13)
+    log(f'_VHFOpt.build {time.perf_counter() - t0:.2f}s: nao_cart={vhfopt.nao} pairs={vhfopt.npairs} '
+        f'classes={vhfopt.ncptype}')
+    dm = init_guess_by_atom(mol)
+    c, ct = vhfopt.coeff_dev()
+    dmc = (c @ torch.as_tensor(dm, device='cuda') @ ct).contiguous()
+    nao = vhfopt.nao
+    vj = torch.empty((1, nao, nao), dtype=torch.float64, device='cuda')
+    vk = torch.empty_like(vj)
+    screen = mjk.Screen(1e-13, args.precision_cutoff, 1)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        mjk.check(lib.mjk_build_jk_all(vhfopt.bp, mjk.ptr(vj), mjk.ptr(vk), mjk.ptr(dmc), nao, 1,
+                                       ctypes.byref(screen), ctypes.c_void_p(stream.cuda_stream)),
+                  'mjk_build_jk_all')
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log('warmup done')
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sec = (t1 - t0) / args.steps
+    st = vhfopt.stats()
+    log(f'J/K build: {sec * 1e3:.1f} ms/step, quartets fp64={st["quartets_fp64"]} fp32={st["quartets_fp32"]}')
+
+    result = {
+        'metric': 'J/K build s/iter (plain FP64, SAD density)',
+        'value': sec, 'unit': 's/iter', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': sec * 1e3, 'higher_is_better': False, 'scaling': 'strong',
+        'vs_baseline': None, 'dtype': 'f64' if args.precision_cutoff == 0 else 'f64+f32',
+        'data': 'synthetic (seeded water cluster, SAD density)',
+        'config': {'workload': args.workload, 'nao': mol.nao, 'nao_cart': nao,
+                   'basis': WORKLOADS[args.workload][3], 'schwarz_cutoff': 1e-13,
+                   'density_screen': True, 'precision_cutoff': args.precision_cutoff,
+                   'quartets': st['quartets_fp64'] + st['quartets_fp32']},
+    }
+    if not args.no_classes:
+        prof = per_class_profile(vhfopt, dmc, vj, vk, screen, torch, lib, mjk)
+        tot_ms = sum(p['ms'] for p in prof)
+        tot_fl = sum(p['flops'] for p in prof)
+        dom = max(prof, key=lambda p: p['ms'])
+        ach = dom['flops'] / (dom['ms'] * 1e-3) / 1e12
+        result['roofline'] = {
+            'bound': 'mfma', 'achieved': ach, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+            'frac': ach / FP64_PEAK_TFLOPS, 'traffic': None,
+            'kernel': f'jk_kernel<double,{dom["class"]}> nprim={dom["nprim"]}',
+            'quartets': dom['quartets'], 'launch_ms': dom['ms'],
+            'note': 'FP64 vector-ALU bound (Rys recursion + digestion); MI355X FP64 vector and '
+                    'matrix peaks are both 78.6 TFLOP/s',
+        }
+        result['build_tflops'] = tot_fl / (tot_ms * 1e-3) / 1e12
+        top = sorted(prof, key=lambda p: -p['ms'])[:8]
+        result['top_classes'] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()
+                                  if k != 'cp'} for p in top]
+        log(f'per-class sum {tot_ms:.1f} ms, {result["build_tflops"]:.2f} TFLOP/s overall; '
+            f'dominant {dom["class"]} {dom["ms"]:.2f} ms {ach:.2f} TFLOP/s')
+    if not args.no_cpu:
+        dmh = dmc[0].cpu().numpy()
+        result['cpu_baseline'] = cpu_baseline(vhfopt, dmh, args.cpu_seconds)
+        log(f'cpu baseline {result["cpu_baseline"]}')
+    if args.scf:
+        result['scf'] = run_scf(mol)
+        log(f'scf {result["scf"]}')
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

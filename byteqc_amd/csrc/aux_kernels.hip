@@ -175,13 +175,12 @@ hipError_t launch_symmetrize(double *v, int n_dm, int nao, double f, hipStream_t
 hipError_t launch_class(int la, int lb, int lc, int ld, bool fp32, const JKArgs &a, hipStream_t st,
                         long *nt)
 {
-    switch (la) {
-    case 0: return launch_la<0>(lb, lc, ld, fp32, a, st, nt);
-    case 1: return launch_la<1>(lb, lc, ld, fp32, a, st, nt);
-    case 2: return launch_la<2>(lb, lc, ld, fp32, a, st, nt);
-    case 3: return launch_la<3>(lb, lc, ld, fp32, a, st, nt);
-    default: return hipErrorInvalidValue;
-    }
+#define MJK_LAB(A, B) \
+    if (la == A && lb == B) return launch_lab<A, B>(lc, ld, fp32, a, st, nt);
+    MJK_LAB(0, 0) MJK_LAB(1, 0) MJK_LAB(1, 1) MJK_LAB(2, 0) MJK_LAB(2, 1)
+    MJK_LAB(2, 2) MJK_LAB(3, 0) MJK_LAB(3, 1) MJK_LAB(3, 2) MJK_LAB(3, 3)
+#undef MJK_LAB
+    return hipErrorInvalidValue;
 }
 
 }  // namespace mjk

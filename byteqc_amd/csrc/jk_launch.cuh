@@ -34,28 +34,27 @@ hipError_t launch(JKArgs a, hipStream_t st, long *ntiles_out)
     return hipSuccess;
 }
 
-template <int LA>
-hipError_t launch_la(int lb, int lc, int ld, bool fp32, const JKArgs &a, hipStream_t st, long *nt)
+template <int LA, int LB>
+hipError_t launch_lab(int lc, int ld, bool fp32, const JKArgs &a, hipStream_t st, long *nt)
 {
-#define MJK_TRY(LB, LC, LD)                                                                    \
+#define MJK_TRY(LC, LD)                                                                        \
     if constexpr (canon(LA, LB, LC, LD)) {                                                     \
-        if (lb == LB && lc == LC && ld == LD)                                                  \
+        if (lc == LC && ld == LD)                                                              \
             return fp32 ? launch<float, LA, LB, LC, LD>(a, st, nt)                             \
                         : launch<double, LA, LB, LC, LD>(a, st, nt);                           \
     }
-#define MJK_TRY_D(LB, LC) MJK_TRY(LB, LC, 0) MJK_TRY(LB, LC, 1) MJK_TRY(LB, LC, 2) MJK_TRY(LB, LC, 3)
-#define MJK_TRY_C(LB) MJK_TRY_D(LB, 0) MJK_TRY_D(LB, 1) MJK_TRY_D(LB, 2) MJK_TRY_D(LB, 3)
-    MJK_TRY_C(0) MJK_TRY_C(1) MJK_TRY_C(2) MJK_TRY_C(3)
-#undef MJK_TRY_C
+#define MJK_TRY_D(LC) MJK_TRY(LC, 0) MJK_TRY(LC, 1) MJK_TRY(LC, 2) MJK_TRY(LC, 3)
+    MJK_TRY_D(0) MJK_TRY_D(1) MJK_TRY_D(2) MJK_TRY_D(3)
 #undef MJK_TRY_D
 #undef MJK_TRY
     return hipErrorInvalidValue;
 }
 
-extern template hipError_t launch_la<0>(int, int, int, bool, const JKArgs &, hipStream_t, long *);
-extern template hipError_t launch_la<1>(int, int, int, bool, const JKArgs &, hipStream_t, long *);
-extern template hipError_t launch_la<2>(int, int, int, bool, const JKArgs &, hipStream_t, long *);
-extern template hipError_t launch_la<3>(int, int, int, bool, const JKArgs &, hipStream_t, long *);
+#define MJK_LAB_DECL(A, B) \
+    extern template hipError_t launch_lab<A, B>(int, int, bool, const JKArgs &, hipStream_t, long *);
+MJK_LAB_DECL(0, 0) MJK_LAB_DECL(1, 0) MJK_LAB_DECL(1, 1) MJK_LAB_DECL(2, 0) MJK_LAB_DECL(2, 1)
+MJK_LAB_DECL(2, 2) MJK_LAB_DECL(3, 0) MJK_LAB_DECL(3, 1) MJK_LAB_DECL(3, 2) MJK_LAB_DECL(3, 3)
+#undef MJK_LAB_DECL
 
 hipError_t launch_class(int la, int lb, int lc, int ld, bool fp32, const JKArgs &a, hipStream_t st, long *nt);
 

@@ -356,7 +356,7 @@ int mjk_build_jk(mjk_basis_prod *bp, double *vj, double *vk, const double *dm, i
     if (!vj && !vk) return MJK_OK;
     hipStream_t st = (hipStream_t)stream;
     const bool dens = screen->density_screen != 0;
-    if (dens) {
+    if (screen->density_screen == 1) {
         hipError_t e = launch_dm_cond(dm, n_dm, nao, bp->ao_loc, bp->nbas, bp->dm_cond, bp->dm_max, st);
         if (e != hipSuccess) return hip_fail(e, "mjk_build_jk: density screen");
     }

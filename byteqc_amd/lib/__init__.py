@@ -30,7 +30,7 @@ class Screen(ctypes.Structure):
 
 EXPORTS = ('mjk_version', 'mjk_last_error', 'mjk_q_cond', 'mjk_init_basis_prod',
            'mjk_del_basis_prod', 'mjk_nao', 'mjk_build_jk', 'mjk_build_jk_all',
-           'mjk_last_stats')
+           'mjk_last_stats', 'mjk_int1e')
 
 _lib = None
 
@@ -58,6 +58,7 @@ def load():
     lib.mjk_build_jk.argtypes = [p, p, p, p, i, i, i, i, ctypes.POINTER(Screen), p]
     lib.mjk_build_jk_all.argtypes = [p, p, p, p, i, i, ctypes.POINTER(Screen), p]
     lib.mjk_last_stats.argtypes = [p, p]
+    lib.mjk_int1e.argtypes = [p, i, p, i, p, p, p, p, i]
     _lib = lib
     return lib
 

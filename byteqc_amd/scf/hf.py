@@ -297,3 +297,6 @@ def _get_jk(mf, mol=None, dm=None, hermi=1, with_j=True, with_k=True, omega=None
         mf._opt_gpu = _VHFOpt(mol, 'int2e').build(mf.direct_scf_tol)
     return get_jk(mol, dm, hermi, mf._opt_gpu, with_j, with_k, omega,
                   precision_cutoff=getattr(mf, 'precision_cutoff', 0.0))
+
+
+from .rhf import RHF, eig  # noqa: E402,F401  (the reference exposes hf.RHF, hf.eig)

@@ -1,0 +1,288 @@
+"""Restricted Hartree-Fock driver around the MI355X J/K builder.
+
+The reference's `RHF` (cuobc/scf/hf.py:272-274) is PySCF's RHF with two methods
+swapped: `get_jk` (J/K on the GPU) and `_eigh` (GPU generalised eigensolver,
+hf.py:253-269).  PySCF is not available here, so this module supplies the loop itself:
+core Hamiltonian from the library's one-electron integrals, SAD or user guess, Pulay
+DIIS, canonical orthogonalisation and eigh on the GPU, and an optional incremental
+Fock build (J/K of the density change, as PySCF's direct SCF does).  Any object with
+a `get_jk(mol, dm, hermi=1, ...)` method can be plugged in: that is the drop-in contract.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .. import lib as mjk
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def int1e(vhfopt, nthreads=0):
+    """(S, T, V) in the caller's AO basis (spherical unless mol.cart)."""
+    lib = mjk.load()
+    n = int(vhfopt.coeff.shape[0])
+    s = np.zeros((n, n))
+    t = np.zeros((n, n))
+    v = np.zeros((n, n))
+    mjk.check(lib.mjk_int1e(mjk.ptr(vhfopt.atm), len(vhfopt.atm), mjk.ptr(vhfopt.bas),
+                            len(vhfopt.bas), mjk.ptr(vhfopt.env), mjk.ptr(s), mjk.ptr(t),
+                            mjk.ptr(v), nthreads), 'mjk_int1e')
+    c = vhfopt.coeff
+    return c.T @ s @ c, c.T @ t @ c, c.T @ v @ c
+
+
+def eig(h, s, threshold=1e-15):
+    """Generalised eigenproblem h c = s c e by canonical orthogonalisation (hf.py:253-265).
+
+    h, s: torch tensors on the GPU.  Eigenvectors of s with eigenvalue below
+    `threshold` are dropped."""
+    torch = _torch()
+    seig, t = torch.linalg.eigh(s)
+    keep = seig >= threshold
+    t = t[:, keep] / torch.sqrt(seig[keep])
+    heff = t.T @ h @ t
+    e, c = torch.linalg.eigh(heff)
+    return e, t @ c
+
+
+class DIIS:
+    """Pulay DIIS on the Fock matrix with the orthogonalised commutator as error."""
+
+    def __init__(self, space=8):
+        self.space = space
+        self.focks, self.errs = [], []
+
+    def update(self, f, err):
+        torch = _torch()
+        self.focks.append(f)
+        self.errs.append(err)
+        if len(self.focks) > self.space:
+            self.focks.pop(0)
+            self.errs.pop(0)
+        n = len(self.focks)
+        if n < 2:
+            return f
+        b = torch.empty((n + 1, n + 1), dtype=torch.float64, device=f.device)
+        b[-1, :] = -1
+        b[:, -1] = -1
+        b[-1, -1] = 0
+        e = torch.stack([x.reshape(-1) for x in self.errs])
+        b[:n, :n] = e @ e.T
+        rhs = torch.zeros(n + 1, dtype=torch.float64, device=f.device)
+        rhs[-1] = -1
+        try:
+            c = torch.linalg.solve(b, rhs)
+        except RuntimeError:
+            c = torch.linalg.lstsq(b, rhs[:, None]).solution[:, 0]
+        return sum(c[i] * self.focks[i] for i in range(n))
+
+
+# ---- superposition of atomic densities ----------------------------------------------
+_ATOM_CACHE = {}
+
+
+def _occupations(e, nelec, tol=1e-3):
+    """Aufbau occupations (per spatial orbital, up to 2) with degenerate levels sharing."""
+    occ = np.zeros(len(e))
+    left = float(nelec)
+    i = 0
+    while left > 1e-12 and i < len(e):
+        j = i + 1
+        while j < len(e) and abs(e[j] - e[i]) < tol:
+            j += 1
+        g = j - i
+        take = min(left, 2.0 * g)
+        occ[i:j] = take / g
+        left -= take
+        i = j
+    return occ
+
+
+def atomic_density(symbol, basis, get_jk=None, max_cycle=60):
+    """Spherically averaged fractional-occupation atomic HF density (spherical AOs)."""
+    key = (symbol, str(basis))
+    if key in _ATOM_CACHE:
+        return _ATOM_CACHE[key]
+    from ..gto import Mole
+    from . import hf
+    atom = Mole([(symbol, (0., 0., 0.))], basis=basis)
+    opt = hf._VHFOpt(atom).build()
+    s, t, v = int1e(opt)
+    h = t + v
+    torch = _torch()
+    hd = torch.as_tensor(h, device='cuda')
+    sd = torch.as_tensor(s, device='cuda')
+    nelec = atom.nelectron
+    e, c = eig(hd, sd)
+    occ = _occupations(e.cpu().numpy(), nelec)
+    dm = (c * torch.as_tensor(occ, device='cuda')) @ c.T
+    diis = DIIS(6)
+    e_old = 0.0
+    for it in range(max_cycle):
+        vj, vk = hf.get_jk(atom, dm, vhfopt=opt, out_device=True)
+        f = hd + vj - 0.5 * vk
+        e_tot = float(0.5 * torch.sum(dm * (hd + f)))
+        err = f @ dm @ sd - sd @ dm @ f
+        f = diis.update(f, err)
+        e, c = eig(f, sd)
+        occ = _occupations(e.cpu().numpy(), nelec)
+        dm_new = (c * torch.as_tensor(occ, device='cuda')) @ c.T
+        if abs(e_tot - e_old) < 1e-9 and float(torch.abs(dm_new - dm).max()) < 1e-7:
+            dm = dm_new
+            break
+        dm, e_old = dm_new, e_tot
+    opt.clear()
+    out = dm.cpu().numpy()
+    _ATOM_CACHE[key] = out
+    return out
+
+
+def init_guess_by_atom(mol):
+    """Block-diagonal superposition of atomic densities in mol's AO order."""
+    n = mol.nao
+    dm = np.zeros((n, n))
+    aoslice = mol.aoslice_by_atom()
+    for ia in range(mol.natm):
+        a0, a1 = aoslice[ia, 2], aoslice[ia, 3]
+        dm[a0:a1, a0:a1] = atomic_density(mol.atom_symbol(ia), mol.basis)
+    return dm
+
+
+class RHF:
+    """Closed-shell SCF.  kernel() returns the converged total energy."""
+
+    def __init__(self, mol, conv_tol=1e-8, conv_tol_grad=None, max_cycle=100, diis_space=8,
+                 direct_scf_tol=1e-13, precision_cutoff=0.0, incremental=True,
+                 rebuild_every=8, verbose=0, jk=None):
+        self.mol = mol
+        self.conv_tol = conv_tol
+        self.conv_tol_grad = conv_tol_grad if conv_tol_grad is not None else np.sqrt(conv_tol) * 0.1
+        self.max_cycle = max_cycle
+        self.diis_space = diis_space
+        self.direct_scf_tol = direct_scf_tol
+        self.precision_cutoff = precision_cutoff
+        self.incremental = incremental
+        self.rebuild_every = rebuild_every
+        self.verbose = verbose
+        self.jk = jk
+        self._opt_gpu = None
+        self.converged = False
+        self.e_tot = None
+        self.cycles = 0
+        self.history = []
+        self.timings = {}
+
+    # ---- pieces --------------------------------------------------------------------
+    def build(self):
+        from . import hf
+        t0 = time.perf_counter()
+        if self._opt_gpu is None:
+            self._opt_gpu = hf._VHFOpt(self.mol).build(self.direct_scf_tol)
+        t1 = time.perf_counter()
+        s, t, v = int1e(self._opt_gpu)
+        self._s, self._h = s, t + v
+        self.timings.update(vhfopt=t1 - t0, int1e=time.perf_counter() - t1)
+        return self
+
+    def get_ovlp(self):
+        return self._s
+
+    def get_hcore(self):
+        return self._h
+
+    def get_init_guess(self, key='sad'):
+        if key in ('sad', 'atom', 'minao'):
+            return init_guess_by_atom(self.mol)
+        if key in ('1e', 'hcore'):
+            torch = _torch()
+            e, c = eig(torch.as_tensor(self._h, device='cuda'), torch.as_tensor(self._s, device='cuda'))
+            nocc = self.mol.nelectron // 2
+            co = c[:, :nocc]
+            return (2.0 * co @ co.T).cpu().numpy()
+        raise ValueError(key)
+
+    def get_jk(self, mol, dm, hermi=1, with_j=True, with_k=True, omega=None):
+        """J/K of a device density; the hook a replacement builder overrides."""
+        if self.jk is not None:
+            return self.jk(mol, dm, hermi=hermi, with_j=with_j, with_k=with_k)
+        from . import hf
+        return hf.get_jk(mol, dm, hermi, self._opt_gpu, with_j, with_k, omega,
+                         precision_cutoff=self.precision_cutoff, out_device=True)
+
+    def make_rdm1(self, mo_coeff=None, mo_occ=None):
+        torch = _torch()
+        c = self.mo_coeff if mo_coeff is None else mo_coeff
+        nocc = self.mol.nelectron // 2
+        co = c[:, :nocc]
+        return 2.0 * co @ co.T
+
+    def energy_nuc(self):
+        return self.mol.energy_nuc()
+
+    # ---- the loop ------------------------------------------------------------------
+    def kernel(self, dm0=None, init_guess='sad'):
+        torch = _torch()
+        if self._opt_gpu is None or not hasattr(self, '_h'):
+            self.build()
+        mol = self.mol
+        t_start = time.perf_counter()
+        if dm0 is None:
+            dm0 = self.get_init_guess(init_guess)
+        t_guess = time.perf_counter()
+        h = torch.as_tensor(self._h, device='cuda')
+        s = torch.as_tensor(self._s, device='cuda')
+        dm = torch.as_tensor(np.asarray(dm0), dtype=torch.float64, device='cuda')
+        enuc = self.energy_nuc()
+        nocc = mol.nelectron // 2
+        diis = DIIS(self.diis_space)
+        vj = vk = None
+        dm_last = None
+        e_last = None
+        self.history = []
+        self.converged = False
+        since_full = 0
+        t_jk = 0.0
+        for cycle in range(self.max_cycle):
+            tj = time.perf_counter()
+            if self.incremental and dm_last is not None and since_full < self.rebuild_every:
+                dj, dk = self.get_jk(mol, dm - dm_last)
+                vj = vj + dj
+                vk = vk + dk
+                since_full += 1
+            else:
+                vj, vk = self.get_jk(mol, dm)
+                since_full = 0
+            torch.cuda.synchronize()
+            t_jk += time.perf_counter() - tj
+            f = h + vj - 0.5 * vk
+            e_tot = float(0.5 * torch.sum(dm * (h + f))) + enuc
+            err = f @ dm @ s - s @ dm @ f
+            gnorm = float(torch.abs(err).max())
+            de = None if e_last is None else e_tot - e_last
+            self.history.append({'cycle': cycle, 'e_tot': e_tot, 'delta_e': de, 'grad': gnorm,
+                                 't': time.perf_counter() - t_start})
+            if self.verbose:
+                print(f'cycle {cycle}: E = {e_tot:.12f}  dE = {de}  |[F,D]| = {gnorm:.2e}', flush=True)
+            if de is not None and abs(de) < self.conv_tol and gnorm < self.conv_tol_grad:
+                self.converged = True
+                e_last = e_tot
+                break
+            fd = diis.update(f, err)
+            mo_e, mo_c = eig(fd, s)
+            dm_last = dm
+            co = mo_c[:, :nocc]
+            dm = 2.0 * co @ co.T
+            e_last = e_tot
+        torch.cuda.synchronize()
+        t_end = time.perf_counter()
+        self.e_tot = e_last
+        self.cycles = len(self.history)
+        self.mo_energy, self.mo_coeff = eig(f, s)
+        self.dm = dm
+        self.timings.update(guess=t_guess - t_start, scf=t_end - t_guess, jk=t_jk, total=t_end - t_start)
+        return self.e_tot

@@ -77,7 +77,9 @@ typedef struct {
     double precision_cutoff;  /* quartets with q_ij q_kl dmax below this run in FP32;
                                  0 = everything FP64 */
     int density_screen;       /* 1: dmax is the max |D| over the 6 shell blocks the quartet
-                                 touches (recomputed from dm each build); 0: dmax = 1 */
+                                 touches (recomputed from dm each build); 0: dmax = 1;
+                                 2 (mjk_build_jk only): reuse the block maxima computed by
+                                 the previous call on this cache */
     int reserved[3];
 } mjk_screen;
 
@@ -96,6 +98,13 @@ int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *d
 /* Counters of the last mjk_build_jk_all on this cache (host values, synchronises):
  * out[0] quartets computed in FP64, out[1] in FP32, out[2] kernel launches. */
 int mjk_last_stats(mjk_basis_prod *bp, double *out);
+
+/* Overlap, kinetic and nuclear-attraction matrices (host arrays, nao x nao each) of a
+ * Cartesian segmented basis, on nthreads host threads (<= 0: all cores).  The SCF
+ * driver's one-electron part; it replaces PySCF's int1e_ovlp / int1e_kin / int1e_nuc
+ * that the reference's RHF inherits (cuobc/scf/hf.py:272-274). */
+int mjk_int1e(const int32_t *atm, int natm, const int32_t *bas, int nbas, const double *env,
+              double *s, double *t, double *v, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -53,7 +53,9 @@ def lib():
         _LIB.oracle_md_shell_quartet.argtypes = [p, p, p, i, i, i, i, p]
         _LIB.oracle_rys_shell_quartet.argtypes = [p, p, p, i, i, i, i, p]
         _LIB.oracle_q_cond.argtypes = [p, i, p, i, p, p, i]
+        _LIB.oracle_md_int1e.argtypes = [p, i, p, i, p, p, p, p]
         _LIB.oracle_rys_jk.argtypes = [p, i, p, i, p, p, d, p, i, i, p, p, i]
+        _LIB.oracle_rys_jk_rows.argtypes = [p, i, p, i, p, p, d, p, i, i, p, p, i, i, i]
     return _LIB
 
 
@@ -166,6 +168,22 @@ def md_eri_cart(sb: SegBasis):
     return eri
 
 
+def md_int1e_cart(sb: SegBasis):
+    n = sb.nao
+    s, t, v = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    err = lib().oracle_md_int1e(_ptr(sb.atm), len(sb.atm), _ptr(sb.bas), sb.nbas, _ptr(sb.env),
+                                _ptr(s), _ptr(t), _ptr(v))
+    if err:
+        raise RuntimeError('oracle_md_int1e failed')
+    return s, t, v
+
+
+def md_int1e_sph(mol):
+    sb = seg_basis(mol, replica=True)
+    c = sb.coeff
+    return tuple(c.T @ m @ c for m in md_int1e_cart(sb))
+
+
 def md_eri_sph(mol):
     sb = seg_basis(mol, replica=True)
     eri = md_eri_cart(sb)
@@ -199,10 +217,10 @@ def rys_jk_cart(sb: SegBasis, dm_cart, with_j=True, with_k=True, cutoff=1e-13, n
     vk = np.zeros_like(dms) if with_k else None
     if cutoff is not None and q is None:
         q = q_cond(sb, nthreads)
-    err = lib().oracle_rys_jk(_ptr(sb.atm), len(sb.atm), _ptr(sb.bas), sb.nbas, _ptr(sb.env),
-                              _ptr(q) if cutoff is not None else None,
-                              float(cutoff or 0.), _ptr(dms), n_dm, sb.nao, _ptr(vj), _ptr(vk),
-                              nthreads)
+    err = lib().oracle_rys_jk_rows(_ptr(sb.atm), len(sb.atm), _ptr(sb.bas), sb.nbas, _ptr(sb.env),
+                                   _ptr(q) if cutoff is not None else None,
+                                   float(cutoff or 0.), _ptr(dms), n_dm, sb.nao, _ptr(vj), _ptr(vk),
+                                   nthreads, int(stride), int(offset))
     if err:
         raise RuntimeError(f'oracle_rys_jk failed ({err})')
     return vj, vk

@@ -251,3 +251,80 @@ int oracle_md_eri(const int32_t *atm, int natm, const int32_t *bas, int nbas,
     free(ao_loc);
     return err;
 }
+
+/* One-electron integrals by McMurchie-Davidson, for pinning the product's integrals.
+ * S from E^ab_0, V from R_tuv(p, P - C); the kinetic energy uses the symmetric form
+ * T_ab = 1/2 sum_d <d_d a | d_d b> with d/dx (x^i e^(-a x^2)) = i x^(i-1) - 2a x^(i+1),
+ * a different route from the -1/2 <a|lap b> form.  Cartesian basis, nao x nao each. */
+int oracle_md_int1e(const int32_t *atm, int natm, const int32_t *bas, int nbas,
+                    const double *env, double *S, double *T, double *V)
+{
+    int *ao_loc = (int *)malloc(sizeof(int) * (nbas + 1));
+    ao_loc[0] = 0;
+    for (int i = 0; i < nbas; i++) ao_loc[i + 1] = ao_loc[i] + ncart(bas[i * BAS_SLOTS + 1]);
+    int n = ao_loc[nbas];
+    size_t cube = (size_t)RD * RD * RD;
+    for (int i = 0; i < nbas; i++)
+    for (int j = 0; j < nbas; j++) {
+        shell_t A = get_shell(atm, bas, env, i), B = get_shell(atm, bas, env, j);
+        if (A.l > 3 || B.l > 3) { free(ao_loc); return 1; }
+        int ax[15], ay[15], az[15], bx[15], by[15], bz[15];
+        cart_powers(A.l, ax, ay, az);
+        cart_powers(B.l, bx, by, bz);
+        int nfa = ncart(A.l), nfb = ncart(B.l);
+        double *R0 = (double *)malloc(sizeof(double) * cube);
+        double *work = (double *)malloc(sizeof(double) * cube * (A.l + B.l + 1));
+        double norm = fac_sp(A.l) * fac_sp(B.l);
+        for (int fa = 0; fa < nfa; fa++)
+        for (int fb = 0; fb < nfb; fb++) {
+            int I = ao_loc[i] + fa, J = ao_loc[j] + fb;
+            S[I * n + J] = T[I * n + J] = V[I * n + J] = 0.0;
+        }
+        for (int ip = 0; ip < A.nprim; ip++)
+        for (int jp = 0; jp < B.nprim; jp++) {
+            double a = A.exp[ip], b = B.exp[jp], p = a + b;
+            double cc = A.coef[ip] * B.coef[jp] * norm;
+            double E[3][EDIM][EDIM][ETDIM], P[3];
+            for (int d = 0; d < 3; d++) {
+                P[d] = (a * A.r[d] + b * B.r[d]) / p;
+                hermite_e(A.l + 1, B.l + 1, a, b, A.r[d], B.r[d], E[d]);
+            }
+            double sq = sqrt(M_PI / p);
+            #define S1(d, ii, jj) (((ii) < 0 || (jj) < 0) ? 0.0 : E[d][ii][jj][0] * sq)
+            for (int fa = 0; fa < nfa; fa++)
+            for (int fb = 0; fb < nfb; fb++) {
+                int pa[3] = {ax[fa], ay[fa], az[fa]}, pb[3] = {bx[fb], by[fb], bz[fb]};
+                double s[3], k[3];
+                for (int d = 0; d < 3; d++) {
+                    int ii = pa[d], jj = pb[d];
+                    s[d] = S1(d, ii, jj);
+                    k[d] = 0.5 * (ii * jj * S1(d, ii - 1, jj - 1) - 2.0 * b * ii * S1(d, ii - 1, jj + 1)
+                                  - 2.0 * a * jj * S1(d, ii + 1, jj - 1) + 4.0 * a * b * S1(d, ii + 1, jj + 1));
+                }
+                int I = ao_loc[i] + fa, J = ao_loc[j] + fb;
+                S[I * n + J] += cc * s[0] * s[1] * s[2];
+                T[I * n + J] += cc * (k[0] * s[1] * s[2] + s[0] * k[1] * s[2] + s[0] * s[1] * k[2]);
+            }
+            #undef S1
+            for (int c = 0; c < natm; c++) {
+                const double *rc = env + atm[c * ATM_SLOTS + 1];
+                hermite_r(A.l + B.l, p, P[0] - rc[0], P[1] - rc[1], P[2] - rc[2], R0, work);
+                double f = -atm[c * ATM_SLOTS] * 2.0 * M_PI / p * cc;
+                for (int fa = 0; fa < nfa; fa++)
+                for (int fb = 0; fb < nfb; fb++) {
+                    double s = 0.0;
+                    for (int t = 0; t <= ax[fa] + bx[fb]; t++)
+                    for (int u = 0; u <= ay[fa] + by[fb]; u++)
+                    for (int v = 0; v <= az[fa] + bz[fb]; v++)
+                        s += E[0][ax[fa]][bx[fb]][t] * E[1][ay[fa]][by[fb]][u] * E[2][az[fa]][bz[fb]][v]
+                             * R0[ridx(t, u, v)];
+                    V[(ao_loc[i] + fa) * n + ao_loc[j] + fb] += f * s;
+                }
+            }
+        }
+        free(R0);
+        free(work);
+    }
+    free(ao_loc);
+    return 0;
+}

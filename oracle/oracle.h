@@ -22,6 +22,10 @@ void oracle_rys_roots(int n, double T, double *x, double *w);
 /* Same, straight from the discretised Stieltjes procedure (no table). */
 void oracle_rys_roots_direct(int n, double T, double *x, double *w);
 
+/* Cartesian overlap, kinetic and nuclear-attraction matrices (nao x nao each). */
+int oracle_md_int1e(const int32_t *atm, int natm, const int32_t *bas, int nbas,
+                    const double *env, double *S, double *T, double *V);
+
 /* Full Cartesian ERI tensor (nao^4, row-major (ij|kl)) by McMurchie-Davidson. */
 int oracle_md_eri(const int32_t *atm, int natm, const int32_t *bas, int nbas,
                   const double *env, double *eri);
@@ -45,6 +49,13 @@ int oracle_rys_jk(const int32_t *atm, int natm, const int32_t *bas, int nbas,
                   const double *env, const double *q, double cutoff,
                   const double *dm, int n_dm, int nao, double *vj, double *vk,
                   int nthreads);
+
+/* Same, restricted to bra pair rows pij with pij % stride == offset (pairs enumerated
+ * i >= j in shell order), for timing a systematic sample of the quartet work. */
+int oracle_rys_jk_rows(const int32_t *atm, int natm, const int32_t *bas, int nbas,
+                       const double *env, const double *q, double cutoff,
+                       const double *dm, int n_dm, int nao, double *vj, double *vk,
+                       int nthreads, int stride, int offset);
 
 #ifdef __cplusplus
 }

@@ -229,6 +229,15 @@ int oracle_rys_jk(const int32_t *atm, int natm, const int32_t *bas, int nbas,
                   const double *dm, int n_dm, int nao, double *vj, double *vk,
                   int nthreads)
 {
+    return oracle_rys_jk_rows(atm, natm, bas, nbas, env, q, cutoff, dm, n_dm, nao, vj, vk,
+                              nthreads, 1, 0);
+}
+
+int oracle_rys_jk_rows(const int32_t *atm, int natm, const int32_t *bas, int nbas,
+                       const double *env, const double *q, double cutoff,
+                       const double *dm, int n_dm, int nao, double *vj, double *vk,
+                       int nthreads, int stride, int offset)
+{
     (void)natm;
     int *ao_loc = make_ao_loc(bas, nbas);
     if (ao_loc[nbas] != nao) { free(ao_loc); return 2; }
@@ -250,6 +259,7 @@ int oracle_rys_jk(const int32_t *atm, int natm, const int32_t *bas, int nbas,
         double *myk = acck ? acck + (size_t)tid * n_dm * nn : NULL;
         #pragma omp for schedule(dynamic, 1)
         for (int pij = npair - 1; pij >= 0; pij--) {
+            if (stride > 1 && pij % stride != offset) continue;
             int i = pi[pij], j = pj[pij];
             for (int pkl = 0; pkl <= pij; pkl++) {
                 int k = pi[pkl], l = pj[pkl];
