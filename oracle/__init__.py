@@ -209,8 +209,9 @@ def q_cond(sb: SegBasis, nthreads=0):
 
 
 def rys_jk_cart(sb: SegBasis, dm_cart, with_j=True, with_k=True, cutoff=1e-13, nthreads=0,
-                q=None):
-    """J/K in the segmented Cartesian basis.  cutoff=None: no screening."""
+                q=None, stride=1, offset=0):
+    """J/K in the segmented Cartesian basis.  cutoff=None: no screening.
+    stride > 1 computes only bra pair rows pij % stride == offset (timing samples)."""
     dms = np.ascontiguousarray(dm_cart, dtype=np.float64).reshape(-1, sb.nao, sb.nao)
     n_dm = dms.shape[0]
     vj = np.zeros_like(dms) if with_j else None
