@@ -2,9 +2,12 @@
 """Benchmark: direct-SCF J/K build on one MI355X.
 
 Workload (default): 64 seeded water molecules in a 16 Angstrom box, def2-TZVP (2752
-spherical / 3072 Cartesian AOs), SAD density, Schwarz cutoff 1e-13 with density-weighted
-screening, plain FP64.  One step = one full J/K build (mjk_build_jk_all) with the
-density already in HBM; `value` is seconds per build (lower is better).
+spherical / 3072 Cartesian AOs), plain FP64, Schwarz cutoff 1e-13.  The density follows
+the reference's J/K harness (cuobc/test/jk.py:7815-7825): numpy.random.seed(2341),
+dm = rand(nao, nao) + its transpose, prescreen=False (no density-weighted screening);
+--density sad uses the SAD guess with density screening instead.  One step = one full
+J/K build (mjk_build_jk_all) with the density already in HBM; `value` is seconds per
+build (lower is better).
 
 Also reported:
   roofline      the most expensive class kernel, timed per launch with HIP events on the
@@ -123,10 +126,12 @@ def main(argv=None):
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--workload', default='h2o64-def2tzvp')
     ap.add_argument('--precision-cutoff', type=float, default=0.0)
+    ap.add_argument('--density', choices=['random', 'sad'], default='random')
     ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-classes', action='store_true')
     ap.add_argument('--scf', action='store_true')
+    ap.add_argument('--classes-out', default=None, help='write every class timing to this JSON file')
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f'bench.py measures one GPU only (got --gpus {args.gpus}); the J/K build has no '
@@ -148,13 +153,20 @@ def main(argv=None):
     vhfopt = hf._VHFOpt(mol).build(1e-13)
     log(f'_VHFOpt.build {time.perf_counter() - t0:.2f}s: nao_cart={vhfopt.nao} pairs={vhfopt.npairs} '
         f'classes={vhfopt.ncptype}')
-    dm = init_guess_by_atom(mol)
+    if args.density == 'random':
+        # the reference harness: numpy.random.seed(2341); dm = rand(nao, nao); dm + dm.T,
+        # get_jk(..., prescreen=False) (cuobc/test/jk.py:7815-7825)
+        rs = np.random.RandomState(2341)
+        dm = rs.rand(mol.nao, mol.nao)
+        dm = dm + dm.T
+    else:
+        dm = init_guess_by_atom(mol)
     c, ct = vhfopt.coeff_dev()
     dmc = (c @ torch.as_tensor(dm, device='cuda') @ ct).contiguous()
     nao = vhfopt.nao
     vj = torch.empty((1, nao, nao), dtype=torch.float64, device='cuda')
     vk = torch.empty_like(vj)
-    screen = mjk.Screen(1e-13, args.precision_cutoff, 1)
+    screen = mjk.Screen(1e-13, args.precision_cutoff, 0 if args.density == 'random' else 1)
     stream = torch.cuda.current_stream()
 
     def step():
@@ -176,14 +188,15 @@ def main(argv=None):
     log(f'J/K build: {sec * 1e3:.1f} ms/step, quartets fp64={st["quartets_fp64"]} fp32={st["quartets_fp32"]}')
 
     result = {
-        'metric': 'J/K build s/iter (plain FP64, SAD density)',
+        'metric': 'J/K build s/iter (plain FP64)',
         'value': sec, 'unit': 's/iter', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': sec * 1e3, 'higher_is_better': False, 'scaling': 'strong',
         'vs_baseline': None, 'dtype': 'f64' if args.precision_cutoff == 0 else 'f64+f32',
-        'data': 'synthetic (seeded water cluster, SAD density)',
+        'data': f'synthetic (seeded water cluster, {args.density} density)',
         'config': {'workload': args.workload, 'nao': mol.nao, 'nao_cart': nao,
                    'basis': WORKLOADS[args.workload][3], 'schwarz_cutoff': 1e-13,
-                   'density_screen': True, 'precision_cutoff': args.precision_cutoff,
+                   'density': args.density, 'density_screen': bool(screen.density_screen),
+                   'precision_cutoff': args.precision_cutoff,
                    'quartets': st['quartets_fp64'] + st['quartets_fp32']},
     }
     if not args.no_classes:
@@ -201,13 +214,17 @@ def main(argv=None):
                     'matrix peaks are both 78.6 TFLOP/s',
         }
         result['build_tflops'] = tot_fl / (tot_ms * 1e-3) / 1e12
+        if args.classes_out:
+            with open(args.classes_out, 'w') as f:
+                json.dump([{k: v for k, v in p.items() if k != 'cp'} for p in
+                           sorted(prof, key=lambda p: -p['ms'])], f, indent=1)
         top = sorted(prof, key=lambda p: -p['ms'])[:8]
         result['top_classes'] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()
                                   if k != 'cp'} for p in top]
         log(f'per-class sum {tot_ms:.1f} ms, {result["build_tflops"]:.2f} TFLOP/s overall; '
             f'dominant {dom["class"]} {dom["ms"]:.2f} ms {ach:.2f} TFLOP/s')
     if not args.no_cpu:
-        dmh = dmc[0].cpu().numpy()
+        dmh = dmc.reshape(nao, nao).cpu().numpy()
         result['cpu_baseline'] = cpu_baseline(vhfopt, dmh, args.cpu_seconds)
         log(f'cpu baseline {result["cpu_baseline"]}')
     if args.scf:

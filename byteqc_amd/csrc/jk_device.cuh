@@ -68,6 +68,8 @@ struct JKArgs {
     int nbas, nao, n_dm;
     const double *dm_max;   // device scalar: global max |D| (null: 1)
     long tile_base;         // first tile of this launch
+    int kc;                 // ket pairs per tile (register and group kernels)
+    double prim_tol;        // primitive quartets with |prefactor| below this are skipped
     int same;               // bra and ket are the same class: p >= q only
     int nb_bra, nb_ket;     // tiles along bra / ket
 };
@@ -205,6 +207,9 @@ __device__ inline void rys2d(R g0, R c00, R c0p, R b00, R b10, R b01, R ab, R cd
 }
 
 // ---- per-class launch configuration ----------------------------------------------
+#ifndef REG_LIMIT
+#define REG_LIMIT 150
+#endif
 template <int LA, int LB, int LC, int LD>
 struct Cfg {
     static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
@@ -212,7 +217,9 @@ struct Cfg {
     static constexpr int NFAB = NFA * NFB, NFCD = NFC * NFD;
     static constexpr int G2 = (LC + 1) * (LD + 1);
     static constexpr int GS = (LA + 1) * (LB + 1) * G2;
-    static constexpr bool REG = 3 * GS + NFAB * NFCD <= 72;
+    // register-resident classes: one root's 2D tables, the quartet's integrals and the
+    // per-lane exchange/Coulomb partial sums fit the 512-entry VGPR+AGPR file
+    static constexpr bool REG = 3 * GS + NFAB * NFCD + NFAB + (NFA + NFB) * NFC <= REG_LIMIT;
     static constexpr int ACC_MAX = 64;
     static constexpr int RPT0 = REG ? NFAB : cmax(1, cmin(NFAB, ACC_MAX / NFCD));
     static constexpr int S = REG ? 1 : (NFAB + RPT0 - 1) / RPT0;
@@ -276,10 +283,6 @@ jk_kernel(const JKArgs a)
     int bi, ki;
     tile_coords<BP, KP>(a, a.tile_base + (long)blockIdx.x, bi, ki);
     const int p0 = bi * BP, q0 = ki * KP;
-    {
-        const double maxb = a.bra.q[p0] * a.ket.q[q0] * (a.dm_max ? *a.dm_max : 1.0);
-        if (maxb <= a.tol || maxb < a.lo) return;
-    }
 
     const int t = threadIdx.x;
     const int qs = t / S, sub = t % S;

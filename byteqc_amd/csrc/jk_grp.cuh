@@ -1,0 +1,337 @@
+// LDS-staged J/K kernel for the larger angular-momentum classes.
+//
+// A workgroup is one wavefront (64 lanes) split into W groups of S lanes; group g owns
+// bra shell pair p = tile * W + g and all groups walk the same chunk of ket pairs.
+// For each ket and primitive quartet the group's lanes evaluate the 3 N_roots 2D
+// tables (one root and one Cartesian direction per lane, looping when S < 3 N_roots)
+// into the group's LDS slice; then each lane assembles the integrals of its RPT bra
+// rows and every ket column in registers.  Digestion:
+//   * J_ij per lane row, kept in registers over the chunk;
+//   * J_kl is uniform over the wave: wave-wide sum, one atomic per element;
+//   * K_ik, K_jk are summed in LDS while consecutive kets share shell k, K_il, K_jl per
+//     quartet, then written with one FP64 atomic per element.
+// The six updates are those of cuobc/lib/gvhf/contract_jk.cu:219-316.
+#pragma once
+#include "jk_device.cuh"
+#include "jk_reg.cuh"
+
+namespace mjk {
+
+
+template <int LA, int LB, int LC, int LD>
+struct GCfg {
+    static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+    static constexpr int NFA = ncart(LA), NFB = ncart(LB), NFC = ncart(LC), NFD = ncart(LD);
+    static constexpr int NFAB = NFA * NFB, NFCD = NFC * NFD;
+    static constexpr int G2 = (LC + 1) * (LD + 1);
+    static constexpr int GS = (LA + 1) * (LB + 1) * G2;
+    static constexpr int TAB = NR * 3 * GS;                      // 2D tables per group
+    static constexpr int KIK = 0, KJK = NFA * NFC, KIL = KJK + NFB * NFC, KJL = KIL + NFA * NFD;
+    static constexpr int KBUF = KJL + NFB * NFD;                 // exchange partial sums per group
+    static constexpr int ACC = 72;                               // accumulators per lane
+    static constexpr int LDS_BYTES = 24 * 1024;
+
+    static constexpr int rpt_of(int s) { return (NFAB + s - 1) / s; }
+    static constexpr int w_of(int s) { return 64 / s; }
+    static constexpr bool fits(int s)
+    {
+        return s <= 64 && s <= NFAB && rpt_of(s) * NFCD <= (NFCD > ACC ? NFCD : ACC) &&
+               (w_of(s) == 1 || w_of(s) * (TAB * 8 + KBUF * 8) <= LDS_BYTES);
+    }
+    // lanes per quartet: best use of the 64 lanes' row slots (W NFAB / (64 RPT)), then
+    // more groups per wave
+    static constexpr int pick()
+    {
+        int best = 0, bnum = 0, bden = 1, bw = 0;
+        for (int s = 1; s <= 64; ++s) {
+            if (!fits(s)) continue;
+            const int num = w_of(s) * NFAB, den = 64 * rpt_of(s);
+            if (best == 0 || num * bden > bnum * den || (num * bden == bnum * den && w_of(s) > bw)) {
+                best = s; bnum = num; bden = den; bw = w_of(s);
+            }
+        }
+        return best;
+    }
+    static constexpr bool OK = pick() > 0;
+    static constexpr int S = OK ? pick() : 1;
+    static constexpr int RPT = rpt_of(S);
+    static constexpr int W = w_of(S);
+};
+
+template <typename R, int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(64)
+jk_grp_kernel(const JKArgs a)
+{
+    using C = GCfg<LA, LB, LC, LD>;
+    constexpr int NR = C::NR, NFA = C::NFA, NFB = C::NFB, NFC = C::NFC, NFD = C::NFD;
+    constexpr int NFAB = C::NFAB, NFCD = C::NFCD, GS = C::GS, G2 = C::G2;
+    constexpr int S = C::S, RPT = C::RPT, W = C::W;
+    __shared__ R tab[W][C::TAB];
+    __shared__ double kbuf[W][C::KBUF];
+
+    const long tile = a.tile_base + (long)blockIdx.x;
+    const int bi = (int)(tile / a.nb_ket), ki = (int)(tile % a.nb_ket);
+    const int lane = threadIdx.x;
+    const int g = lane / S, sub = lane % S;
+    const bool lane_ok = g < W;
+    const int gg = lane_ok ? g : 0;
+    const int p = bi * W + g;
+    const int q_begin = ki * a.kc;
+    if (a.same && q_begin > bi * W + W - 1) return;  // the tile lies above the triangle
+    int q_end = q_begin + a.kc;
+    if (q_end > a.ket.npairs) q_end = a.ket.npairs;
+
+    const bool pvalid = lane_ok && p < a.bra.npairs;
+    int ish = 0, jsh = 0;
+    double qp = 0.0;
+    if (pvalid) {
+        ish = a.bra.ish[p];
+        jsh = a.bra.jsh[p];
+        qp = a.bra.q[p];
+    }
+    const double dmax = a.dm_max ? *a.dm_max : 1.0;
+    {
+        double qk = 0.0;
+        for (int q = q_begin + lane; q < q_end; q += 64) qk = fmax(qk, a.ket.q[q]);
+        double qb = qp;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            qk = fmax(qk, __shfl_xor(qk, off, 64));
+            qb = fmax(qb, __shfl_xor(qb, off, 64));
+        }
+        if (!(qb * qk * dmax > a.tol && qb * qk * dmax >= a.lo)) return;  // wave-uniform
+    }
+    const double A[3] = {a.bx[ish], a.by[ish], a.bz[ish]};
+    const double AB[3] = {A[0] - a.bx[jsh], A[1] - a.by[jsh], A[2] - a.bz[jsh]};
+    const int nao = a.nao;
+    const int i0 = a.ao_loc[ish], j0 = a.ao_loc[jsh];
+    const double *D = a.dm;
+    double *VJ = a.vj, *VK = a.vk;
+    const bool doj = VJ != nullptr, dok = VK != nullptr;
+
+    // this lane's bra rows
+    int ox[RPT], oy[RPT], oz[RPT], rowia[RPT], rowjb[RPT];
+    bool rowok[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int row = sub + k * S;
+        rowok[k] = lane_ok && row < NFAB;
+        const int rr = rowok[k] ? row : 0;
+        const int ia = rr / NFB, jb = rr % NFB;
+        rowia[k] = ia;
+        rowjb[k] = jb;
+        ox[k] = (cart_x(LA, ia) * (LB + 1) + cart_x(LB, jb)) * G2;
+        oy[k] = (cart_y(LA, ia) * (LB + 1) + cart_y(LB, jb)) * G2;
+        oz[k] = (cart_z(LA, ia) * (LB + 1) + cart_z(LB, jb)) * G2;
+    }
+    double jij[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) jij[k] = 0.0;
+    for (int x = sub; x < C::KBUF; x += S)
+        if (lane_ok) kbuf[gg][x] = 0.0;
+    int cur_k = -1, cur_k0 = 0;
+    unsigned long long nquart = 0;
+    const int npb = a.bra.nprim, npk = a.ket.nprim;
+    __syncthreads();
+
+    for (int q = q_begin; q < q_end; ++q) {
+        const int ksh = __builtin_amdgcn_readfirstlane(a.ket.ish[q]);
+        const int lsh = __builtin_amdgcn_readfirstlane(a.ket.jsh[q]);
+        const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+        if (ksh != cur_k) {
+            if (dok && cur_k >= 0 && pvalid) {
+                for (int x = sub; x < C::KIL; x += S) {
+                    const double v = kbuf[gg][x];
+                    if (v != 0.0) {
+                        const size_t row = x < C::KJK ? (size_t)(i0 + x / NFC) : (size_t)(j0 + (x - C::KJK) / NFC);
+                        unsafeAtomicAdd(VK + row * nao + cur_k0 + x % NFC, v);
+                        kbuf[gg][x] = 0.0;
+                    }
+                }
+            }
+            cur_k = ksh;
+            cur_k0 = k0;
+            __syncthreads();
+        }
+        bool active = pvalid && (!a.same || q <= p);
+        if (active) {
+            double bound = qp * a.ket.q[q];
+            if (a.dm_cond) {
+                const double *dc = a.dm_cond;
+                const int nb = a.nbas;
+                const double d = fmax(fmax(dc[ish * nb + jsh], dc[ksh * nb + lsh]),
+                                      fmax(fmax(dc[ish * nb + ksh], dc[ish * nb + lsh]),
+                                           fmax(dc[jsh * nb + ksh], dc[jsh * nb + lsh])));
+                bound *= d;
+            }
+            active = bound > a.tol && bound >= a.lo && bound < a.hi;
+        }
+        if (!__builtin_amdgcn_ballot_w64(active)) continue;  // wave-uniform
+        if (active && sub == 0) ++nquart;
+        R acc[RPT][NFCD];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+#pragma unroll
+            for (int c = 0; c < NFCD; ++c) acc[k][c] = (R)0;
+        const double Cc[3] = {a.bx[ksh], a.by[ksh], a.bz[ksh]};
+        const double CD[3] = {Cc[0] - a.bx[lsh], Cc[1] - a.by[lsh], Cc[2] - a.bz[lsh]};
+        const double norm = (a.same && p == q) ? 0.5 * PI52X2 : PI52X2;
+        for (int ip = 0; ip < npb; ++ip) {
+            for (int kq = 0; kq < npk; ++kq) {
+                const int pi_ = p * npb + ip, qi_ = q * npk + kq;
+                const double pref = active ? norm * a.bra.e[pi_] * a.ket.e[qi_] /
+                                                 (a.bra.a[pi_] * a.ket.a[qi_] * sqrt(a.bra.a[pi_] + a.ket.a[qi_]))
+                                           : 0.0;
+                const bool pactive = active && fabs(pref) >= a.prim_tol;
+                if (!__builtin_amdgcn_ballot_w64(pactive)) continue;  // wave-uniform: barriers stay paired
+                if (pactive) {
+                    const double aij = a.bra.a[pi_], akl = a.ket.a[qi_];
+                    const double P[3] = {a.bra.px[pi_], a.bra.py[pi_], a.bra.pz[pi_]};
+                    const double Q[3] = {a.ket.px[qi_], a.ket.py[qi_], a.ket.pz[qi_]};
+                    const double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+                    const double aijkl = aij + akl;
+                    const double T = aij * akl / aijkl * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+                    const double inv2 = 0.5 / aijkl;
+                    for (int it = sub; it < 3 * NR; it += S) {
+                        const int r = it / 3, d = it % 3;
+                        R x, w;
+                        rys_root<R, NR>(T, r, x, w);
+                        const R b00 = x * (R)inv2;
+                        const R b10 = ((R)0.5 - b00 * (R)akl) / (R)aij;
+                        const R b01 = ((R)0.5 - b00 * (R)aij) / (R)akl;
+                        const R tb = (R)2 * b00;
+                        const double Pd = d == 0 ? P[0] : (d == 1 ? P[1] : P[2]);
+                        const double Qd = d == 0 ? Q[0] : (d == 1 ? Q[1] : Q[2]);
+                        const double PQd = d == 0 ? PQ[0] : (d == 1 ? PQ[1] : PQ[2]);
+                        const double Ad = d == 0 ? A[0] : (d == 1 ? A[1] : A[2]);
+                        const double Cd = d == 0 ? Cc[0] : (d == 1 ? Cc[1] : Cc[2]);
+                        const double ABd = d == 0 ? AB[0] : (d == 1 ? AB[1] : AB[2]);
+                        const double CDd = d == 0 ? CD[0] : (d == 1 ? CD[1] : CD[2]);
+                        R g0 = (R)1;
+                        if (d == 2) g0 = w * (R)pref;
+                        rys2d<R, LA, LB, LC, LD>(g0, (R)(Pd - Ad) - tb * (R)akl * (R)PQd,
+                                                 (R)(Qd - Cd) + tb * (R)aij * (R)PQd, b00, b10, b01,
+                                                 (R)ABd, (R)CDd, &tab[gg][(r * 3 + d) * GS]);
+                    }
+                }
+                __syncthreads();
+                if (pactive) {
+#pragma unroll 1
+                    for (int r = 0; r < NR; ++r) {
+                        const R *gx = &tab[gg][(r * 3) * GS], *gy = gx + GS, *gz = gx + 2 * GS;
+#pragma unroll
+                        for (int k = 0; k < RPT; ++k) {
+                            R xs[G2], ys[G2], zs[G2];
+#pragma unroll
+                            for (int e = 0; e < G2; ++e) {
+                                xs[e] = gx[ox[k] + e];
+                                ys[e] = gy[oy[k] + e];
+                                zs[e] = gz[oz[k] + e];
+                            }
+#pragma unroll
+                            for (int c = 0; c < NFCD; ++c) {
+                                const int kc = c / NFD, ld = c % NFD;
+                                const int cx = cart_x(LC, kc) * (LD + 1) + cart_x(LD, ld);
+                                const int cy = cart_y(LC, kc) * (LD + 1) + cart_y(LD, ld);
+                                const int cz = cart_z(LC, kc) * (LD + 1) + cart_z(LD, ld);
+                                acc[k][c] += xs[cx] * ys[cy] * zs[cz];
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // ---- digestion
+        if (doj) {
+            double jkl[NFCD];
+#pragma unroll
+            for (int c = 0; c < NFCD; ++c) jkl[c] = 0.0;
+            if (active) {
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    if (!rowok[k]) continue;
+                    const double dij = D[(size_t)(i0 + rowia[k]) * nao + j0 + rowjb[k]];
+#pragma unroll
+                    for (int c = 0; c < NFCD; ++c) {
+                        const double v = (double)acc[k][c];
+                        jij[k] += v * D[(size_t)(k0 + c / NFD) * nao + l0 + c % NFD];
+                        jkl[c] += v * dij;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NFCD; ++c) {
+                const double s = wave_sum(jkl[c]);
+                if (lane == 0) unsafeAtomicAdd(VJ + (size_t)(k0 + c / NFD) * nao + l0 + c % NFD, s);
+            }
+        }
+        if (dok) {
+            if (active) {
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    if (!rowok[k]) continue;
+                    const int ia = rowia[k], jb = rowjb[k];
+                    const size_t ri = (size_t)(i0 + ia) * nao, rj = (size_t)(j0 + jb) * nao;
+                    double pik[NFC], pjk[NFC], pil[NFD], pjl[NFD];
+#pragma unroll
+                    for (int x = 0; x < NFC; ++x) { pik[x] = 0.0; pjk[x] = 0.0; }
+#pragma unroll
+                    for (int x = 0; x < NFD; ++x) { pil[x] = 0.0; pjl[x] = 0.0; }
+#pragma unroll
+                    for (int kc = 0; kc < NFC; ++kc) {
+                        const double djk = D[rj + k0 + kc], dik = D[ri + k0 + kc];
+#pragma unroll
+                        for (int ld = 0; ld < NFD; ++ld) {
+                            const double v = (double)acc[k][kc * NFD + ld];
+                            pik[kc] += v * D[rj + l0 + ld];
+                            pjk[kc] += v * D[ri + l0 + ld];
+                            pil[ld] += v * djk;
+                            pjl[ld] += v * dik;
+                        }
+                    }
+#pragma unroll
+                    for (int x = 0; x < NFC; ++x) {
+                        atomicAdd(&kbuf[gg][C::KIK + ia * NFC + x], pik[x]);
+                        atomicAdd(&kbuf[gg][C::KJK + jb * NFC + x], pjk[x]);
+                    }
+#pragma unroll
+                    for (int x = 0; x < NFD; ++x) {
+                        atomicAdd(&kbuf[gg][C::KIL + ia * NFD + x], pil[x]);
+                        atomicAdd(&kbuf[gg][C::KJL + jb * NFD + x], pjl[x]);
+                    }
+                }
+            }
+            __syncthreads();
+            if (active) {
+                for (int x = C::KIL + sub; x < C::KBUF; x += S) {
+                    const size_t row = x < C::KJL ? (size_t)(i0 + (x - C::KIL) / NFD) : (size_t)(j0 + (x - C::KJL) / NFD);
+                    unsafeAtomicAdd(VK + row * nao + l0 + (x - C::KIL) % NFD, kbuf[gg][x]);
+                    kbuf[gg][x] = 0.0;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (dok && cur_k >= 0 && pvalid) {
+        for (int x = sub; x < C::KIL; x += S) {
+            const double v = kbuf[gg][x];
+            if (v != 0.0) {
+                const size_t row = x < C::KJK ? (size_t)(i0 + x / NFC) : (size_t)(j0 + (x - C::KJK) / NFC);
+                unsafeAtomicAdd(VK + row * nao + cur_k0 + x % NFC, v);
+            }
+        }
+    }
+    if (doj && pvalid) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (rowok[k] && jij[k] != 0.0)
+                unsafeAtomicAdd(VJ + (size_t)(i0 + rowia[k]) * nao + j0 + rowjb[k], jij[k]);
+    }
+    if (a.counter) {
+        const double cnt = wave_sum((double)nquart);
+        if (lane == 0 && cnt > 0) atomicAdd(a.counter, (unsigned long long)cnt);
+    }
+}
+
+}  // namespace mjk

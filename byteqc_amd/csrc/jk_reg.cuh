@@ -1,0 +1,305 @@
+// Register-resident J/K kernel for the small angular-momentum classes.
+//
+// One lane owns one bra shell pair for the whole launch tile; all lanes of the
+// workgroup walk the same sequence of ket pairs (a chunk of the ket class), so every
+// ket quantity is uniform across the wave:
+//   * the quartet's Cartesian integrals live in registers (2D tables of one root at a
+//     time, contracted over primitives and roots);
+//   * J_ij accumulates per lane over the whole chunk and is flushed once;
+//   * J_kl is uniform per ket: a wave-wide sum, then one atomic per element per wave;
+//   * K_ik and K_jk accumulate per lane while consecutive kets share shell k (ket pairs
+//     are ordered by (k, l)) and are flushed when k changes;
+//   * K_il and K_jl are flushed per quartet.
+// The six updates are those of cuobc/lib/gvhf/contract_jk.cu:219-316.
+#pragma once
+#include "jk_device.cuh"
+
+namespace mjk {
+
+constexpr int REG_BRA = 256;  // bra pairs (lanes) per tile
+constexpr int REG_KC = 128;   // ket pairs per tile
+
+// Tiles of a same-class launch with ket chunks of kc pairs (kc divides REG_BRA): bra block
+// b needs ket chunks 0 .. min(nb_ket, (b+1) r) - 1, r = REG_BRA / kc.  Returns the number of
+// tiles before bra block bi.
+__host__ __device__ inline long reg_tri_tiles(long bi, long nb_ket, long kc)
+{
+    const long r = REG_BRA / kc;
+    const long B = nb_ket / r;
+    const long m = bi < B ? bi : B;
+    return r * m * (m + 1) / 2 + (bi > B ? (bi - B) * nb_ket : 0);
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename R, int LA, int LB, int LC, int LD>
+__global__ void __launch_bounds__(REG_BRA)
+jk_reg_kernel(const JKArgs a)
+{
+    constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+    constexpr int NFA = ncart(LA), NFB = ncart(LB), NFC = ncart(LC), NFD = ncart(LD);
+    constexpr int NFAB = NFA * NFB, NFCD = NFC * NFD;
+    constexpr int G2 = (LC + 1) * (LD + 1);
+    constexpr int GS = (LA + 1) * (LB + 1) * G2;
+
+    // ---- tile
+    const long tile = a.tile_base + (long)blockIdx.x;
+    int bi, ki;
+    if (!a.same) {
+        bi = (int)(tile / a.nb_ket);
+        ki = (int)(tile % a.nb_ket);
+    } else {
+        long lo = 0, hi = a.nb_bra;
+        while (hi - lo > 1) {
+            const long mid = (lo + hi) / 2;
+            if (reg_tri_tiles(mid, a.nb_ket, a.kc) <= tile) lo = mid; else hi = mid;
+        }
+        bi = (int)lo;
+        ki = (int)(tile - reg_tri_tiles(lo, a.nb_ket, a.kc));
+    }
+    const int t = threadIdx.x;
+    const int p = bi * REG_BRA + t;
+    const int q_begin = ki * a.kc;
+    int q_end = q_begin + a.kc;
+    if (q_end > a.ket.npairs) q_end = a.ket.npairs;
+
+    const bool pvalid = p < a.bra.npairs;
+    int ish = 0, jsh = 0;
+    double qp = 0.0;
+    if (pvalid) {
+        ish = a.bra.ish[p];
+        jsh = a.bra.jsh[p];
+        qp = a.bra.q[p];
+    }
+    const double dmax = a.dm_max ? *a.dm_max : 1.0;
+    // cheap tile test: the largest bound any quartet of the tile can reach
+    {
+        double qk = 0.0;
+        for (int q = q_begin + (t & 63); q < q_end; q += 64) qk = fmax(qk, a.ket.q[q]);
+        double qb = qp;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            qk = fmax(qk, __shfl_xor(qk, off, 64));
+            qb = fmax(qb, __shfl_xor(qb, off, 64));
+        }
+        const int live = qb * qk * dmax > a.tol && qb * qk * dmax >= a.lo;
+        if (!__syncthreads_or(live)) return;
+    }
+
+    const double A[3] = {a.bx[ish], a.by[ish], a.bz[ish]};
+    const double AB[3] = {A[0] - a.bx[jsh], A[1] - a.by[jsh], A[2] - a.bz[jsh]};
+    const int nao = a.nao;
+    const int i0 = a.ao_loc[ish], j0 = a.ao_loc[jsh];
+    const double *D = a.dm;
+    double *VJ = a.vj, *VK = a.vk;
+    const bool doj = VJ != nullptr, dok = VK != nullptr;
+
+    double jij[NFAB];
+    double kik[NFA][NFC], kjk[NFB][NFC];
+#pragma unroll
+    for (int r = 0; r < NFAB; ++r) jij[r] = 0.0;
+#pragma unroll
+    for (int x = 0; x < NFA; ++x)
+#pragma unroll
+        for (int c = 0; c < NFC; ++c) kik[x][c] = 0.0;
+#pragma unroll
+    for (int x = 0; x < NFB; ++x)
+#pragma unroll
+        for (int c = 0; c < NFC; ++c) kjk[x][c] = 0.0;
+    int cur_k = -1, cur_k0 = 0;
+    bool kdirty = false;
+    unsigned long long nquart = 0;
+    const int npb = a.bra.nprim, npk = a.ket.nprim;
+
+    for (int q = q_begin; q < q_end; ++q) {
+        const int ksh = __builtin_amdgcn_readfirstlane(a.ket.ish[q]);
+        const int lsh = __builtin_amdgcn_readfirstlane(a.ket.jsh[q]);
+        const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+        if (ksh != cur_k) {
+            if (dok && kdirty) {
+#pragma unroll
+                for (int x = 0; x < NFA; ++x)
+#pragma unroll
+                    for (int c = 0; c < NFC; ++c) {
+                        unsafeAtomicAdd(VK + (size_t)(i0 + x) * nao + cur_k0 + c, kik[x][c]);
+                        kik[x][c] = 0.0;
+                    }
+#pragma unroll
+                for (int x = 0; x < NFB; ++x)
+#pragma unroll
+                    for (int c = 0; c < NFC; ++c) {
+                        unsafeAtomicAdd(VK + (size_t)(j0 + x) * nao + cur_k0 + c, kjk[x][c]);
+                        kjk[x][c] = 0.0;
+                    }
+            }
+            kdirty = false;
+            cur_k = ksh;
+            cur_k0 = k0;
+        }
+        bool active = pvalid && (!a.same || q <= p);
+        if (active) {
+            double bound = qp * a.ket.q[q];
+            if (a.dm_cond) {
+                const double *dc = a.dm_cond;
+                const int nb = a.nbas;
+                const double d = fmax(fmax(dc[ish * nb + jsh], dc[ksh * nb + lsh]),
+                                      fmax(fmax(dc[ish * nb + ksh], dc[ish * nb + lsh]),
+                                           fmax(dc[jsh * nb + ksh], dc[jsh * nb + lsh])));
+                bound *= d;
+            }
+            active = bound > a.tol && bound >= a.lo && bound < a.hi;
+        }
+        if (!__builtin_amdgcn_ballot_w64(active)) continue;  // wave-uniform skip
+        R acc[NFAB][NFCD];
+#pragma unroll
+        for (int r = 0; r < NFAB; ++r)
+#pragma unroll
+            for (int c = 0; c < NFCD; ++c) acc[r][c] = (R)0;
+        if (active) {
+            ++nquart;
+            const double Cc[3] = {a.bx[ksh], a.by[ksh], a.bz[ksh]};
+            const double CD[3] = {Cc[0] - a.bx[lsh], Cc[1] - a.by[lsh], Cc[2] - a.bz[lsh]};
+            const double norm = (a.same && p == q) ? 0.5 * PI52X2 : PI52X2;
+            for (int ip = 0; ip < npb; ++ip) {
+                const int pi_ = p * npb + ip;
+                const double aij = a.bra.a[pi_], eij = a.bra.e[pi_];
+                const double P[3] = {a.bra.px[pi_], a.bra.py[pi_], a.bra.pz[pi_]};
+                for (int kq = 0; kq < npk; ++kq) {
+                    const int qi_ = q * npk + kq;
+                    const double akl = a.ket.a[qi_], ekl = a.ket.e[qi_];
+                    const double Q[3] = {a.ket.px[qi_], a.ket.py[qi_], a.ket.pz[qi_]};
+                    const double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+                    const double aijkl = aij + akl;
+                    const double T = aij * akl / aijkl * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+                    const double pref = norm * eij * ekl / (aij * akl * sqrt(aijkl));
+                    if (fabs(pref) < a.prim_tol) continue;
+                    const double inv2 = 0.5 / aijkl;
+#pragma unroll 1
+                    for (int r = 0; r < NR; ++r) {
+                        R x, w;
+                        rys_root<R, NR>(T, r, x, w);
+                        const R b00 = x * (R)inv2;
+                        const R b10 = ((R)0.5 - b00 * (R)akl) / (R)aij;
+                        const R b01 = ((R)0.5 - b00 * (R)aij) / (R)akl;
+                        const R tb = (R)2 * b00;
+                        R gx[GS], gy[GS], gz[GS];
+                        rys2d<R, LA, LB, LC, LD>((R)1, (R)(P[0] - A[0]) - tb * (R)akl * (R)PQ[0],
+                                                 (R)(Q[0] - Cc[0]) + tb * (R)aij * (R)PQ[0],
+                                                 b00, b10, b01, (R)AB[0], (R)CD[0], gx);
+                        rys2d<R, LA, LB, LC, LD>((R)1, (R)(P[1] - A[1]) - tb * (R)akl * (R)PQ[1],
+                                                 (R)(Q[1] - Cc[1]) + tb * (R)aij * (R)PQ[1],
+                                                 b00, b10, b01, (R)AB[1], (R)CD[1], gy);
+                        rys2d<R, LA, LB, LC, LD>(w * (R)pref, (R)(P[2] - A[2]) - tb * (R)akl * (R)PQ[2],
+                                                 (R)(Q[2] - Cc[2]) + tb * (R)aij * (R)PQ[2],
+                                                 b00, b10, b01, (R)AB[2], (R)CD[2], gz);
+#pragma unroll
+                        for (int row = 0; row < NFAB; ++row) {
+                            const int ia = row / NFB, jb = row % NFB;
+                            const int ox = (cart_x(LA, ia) * (LB + 1) + cart_x(LB, jb)) * G2;
+                            const int oy = (cart_y(LA, ia) * (LB + 1) + cart_y(LB, jb)) * G2;
+                            const int oz = (cart_z(LA, ia) * (LB + 1) + cart_z(LB, jb)) * G2;
+#pragma unroll
+                            for (int c = 0; c < NFCD; ++c) {
+                                const int kc = c / NFD, ld = c % NFD;
+                                const int cx = cart_x(LC, kc) * (LD + 1) + cart_x(LD, ld);
+                                const int cy = cart_y(LC, kc) * (LD + 1) + cart_y(LD, ld);
+                                const int cz = cart_z(LC, kc) * (LD + 1) + cart_z(LD, ld);
+                                acc[row][c] += gx[ox + cx] * gy[oy + cy] * gz[oz + cz];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // ---- digestion (inactive lanes hold zeros)
+        if (doj) {
+            double jkl[NFCD];
+#pragma unroll
+            for (int c = 0; c < NFCD; ++c) jkl[c] = 0.0;
+            if (active) {
+#pragma unroll
+                for (int r = 0; r < NFAB; ++r) {
+                    const double dij = D[(size_t)(i0 + r / NFB) * nao + j0 + r % NFB];
+#pragma unroll
+                    for (int c = 0; c < NFCD; ++c) {
+                        const double dkl = D[(size_t)(k0 + c / NFD) * nao + l0 + c % NFD];
+                        jij[r] += (double)acc[r][c] * dkl;
+                        jkl[c] += (double)acc[r][c] * dij;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NFCD; ++c) {
+                const double s = wave_sum(jkl[c]);
+                if ((t & 63) == 0) unsafeAtomicAdd(VJ + (size_t)(k0 + c / NFD) * nao + l0 + c % NFD, s);
+            }
+        }
+        if (dok && active) {
+            kdirty = true;
+            const size_t ri = (size_t)i0 * nao, rj = (size_t)j0 * nao;
+            double kil[NFA][NFD], kjl[NFB][NFD];
+#pragma unroll
+            for (int x = 0; x < NFA; ++x)
+#pragma unroll
+                for (int d = 0; d < NFD; ++d) kil[x][d] = 0.0;
+#pragma unroll
+            for (int x = 0; x < NFB; ++x)
+#pragma unroll
+                for (int d = 0; d < NFD; ++d) kjl[x][d] = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < NFA; ++ia)
+#pragma unroll
+                for (int jb = 0; jb < NFB; ++jb) {
+                    const size_t rowi = ri + (size_t)ia * nao, rowj = rj + (size_t)jb * nao;
+#pragma unroll
+                    for (int kc = 0; kc < NFC; ++kc) {
+                        const double djk = D[rowj + k0 + kc], dik = D[rowi + k0 + kc];
+#pragma unroll
+                        for (int ld = 0; ld < NFD; ++ld) {
+                            const double v = (double)acc[ia * NFB + jb][kc * NFD + ld];
+                            const double djl = D[rowj + l0 + ld], dil = D[rowi + l0 + ld];
+                            kik[ia][kc] += v * djl;
+                            kjk[jb][kc] += v * dil;
+                            kil[ia][ld] += v * djk;
+                            kjl[jb][ld] += v * dik;
+                        }
+                    }
+                }
+#pragma unroll
+            for (int x = 0; x < NFA; ++x)
+#pragma unroll
+                for (int d = 0; d < NFD; ++d) unsafeAtomicAdd(VK + ri + (size_t)x * nao + l0 + d, kil[x][d]);
+#pragma unroll
+            for (int x = 0; x < NFB; ++x)
+#pragma unroll
+                for (int d = 0; d < NFD; ++d) unsafeAtomicAdd(VK + rj + (size_t)x * nao + l0 + d, kjl[x][d]);
+        }
+    }
+    // ---- final flushes
+    if (dok && kdirty) {
+#pragma unroll
+        for (int x = 0; x < NFA; ++x)
+#pragma unroll
+            for (int c = 0; c < NFC; ++c) unsafeAtomicAdd(VK + (size_t)(i0 + x) * nao + cur_k0 + c, kik[x][c]);
+#pragma unroll
+        for (int x = 0; x < NFB; ++x)
+#pragma unroll
+            for (int c = 0; c < NFC; ++c) unsafeAtomicAdd(VK + (size_t)(j0 + x) * nao + cur_k0 + c, kjk[x][c]);
+    }
+    if (doj && pvalid) {
+#pragma unroll
+        for (int r = 0; r < NFAB; ++r)
+            if (jij[r] != 0.0) unsafeAtomicAdd(VJ + (size_t)(i0 + r / NFB) * nao + j0 + r % NFB, jij[r]);
+    }
+    if (a.counter) {
+        double cnt = wave_sum((double)nquart);
+        if ((t & 63) == 0 && cnt > 0) atomicAdd(a.counter, (unsigned long long)cnt);
+    }
+}
+
+}  // namespace mjk

@@ -215,10 +215,6 @@ int mjk_init_basis_prod(mjk_basis_prod **pbp, double diag_fac, const int32_t *ba
                 delete bp;
                 return fail(MJK_ERR_ARG, "mjk_init_basis_prod: pair class mixes (l, nprim) patterns");
             }
-            if (pr > ct.pair_off && pair_q[pr] > pair_q[pr - 1]) {
-                delete bp;
-                return fail(MJK_ERR_ARG, "mjk_init_basis_prod: pair_q must be sorted descending in each class");
-            }
             pi[pr] = i;
             pj[pr] = j;
             const double *ai = env + bi[PTR_EXP], *aj = env + bj[PTR_EXP];
@@ -306,6 +302,8 @@ int run_class_pair(mjk_basis_prod *bp, double *vj, double *vk, const double *dm,
     a.dm_cond = dens ? bp->dm_cond : nullptr;
     a.dm_max = dens ? (const double *)bp->dm_max : nullptr;
     a.tol = sc.direct_scf_tol;
+    // a primitive quartet whose prefactor is 1e-6 below the quartet cutoff cannot matter
+    a.prim_tol = sc.direct_scf_tol * 1e-6;
     a.nbas = bp->nbas; a.nao = bp->nao; a.n_dm = n_dm;
     a.same = cp_ij == cp_kl;
     const double pc = sc.precision_cutoff;

@@ -88,10 +88,13 @@ def basis_seg_contraction(mol):
 class _VHFOpt:
     """Shell sorting, Schwarz pair lists and the device basis-product cache.
 
-    build() follows cuobc/scf/hf.py:288-408: shells are stably sorted by (l, nprim);
-    for every group pair (p >= q) the shell pairs with Schwarz factor above `cutoff`
-    are kept, sorted by descending factor, lower triangle only on diagonal groups; the
-    pair data goes to the device with e12 halved on same-shell pairs.
+    build() follows cuobc/scf/hf.py:288-408: shells are grouped by (l, nprim); for every
+    group pair (p >= q) the shell pairs with Schwarz factor above `cutoff` are kept,
+    lower triangle only on diagonal groups; the pair data goes to the device with e12
+    halved on same-shell pairs.  Unlike the reference (which sorts pairs by descending
+    factor for its log-q bins), pairs stay in (i, j) order and shells inside a group are
+    ordered along a space-filling curve: the kernels screen each quartet themselves and
+    use the shared shells of neighbouring pairs to cut exchange atomics.
     """
 
     def __init__(self, mol, intor='int2e', prescreen='CVHFnoscreen',
@@ -120,7 +123,9 @@ class _VHFOpt:
             raise NotImplementedError(f'angular momentum above {"spdf"[LMAX_ON_GPU]} is not supported')
         if group_size is not None:
             uniq, counts = _split_l_ctr_groups(uniq, counts, group_size)
-        order = np.argsort(inv, kind='stable')
+        # within each (l, nprim) group, shells follow a Morton curve through space so
+        # that neighbouring pair indices are neighbouring in space
+        order = np.lexsort((_morton_keys(self.atm, self.env, bas), inv))
         nf = (bas[:, ANG_OF] + 1) * (bas[:, ANG_OF] + 2) // 2
         ao_loc = np.append(0, np.cumsum(nf))
         ao_idx = np.concatenate([np.arange(ao_loc[i], ao_loc[i + 1]) for i in order])
@@ -151,11 +156,12 @@ class _VHFOpt:
                 if qg == p and not diag_block_with_triu:
                     mask &= ii >= jj
                 ii, jj, vals = ii[mask], jj[mask], vals[mask]
-                idx = np.argsort(-vals, kind='stable')
-                pair_i.append(ii[idx])
-                pair_j.append(jj[idx])
-                pair_q.append(vals[idx])
-                locs.append(locs[-1] + len(idx))
+                # (i, j) order: consecutive kets share shell k, which the kernels use to
+                # keep exchange partial sums in registers across kets
+                pair_i.append(ii)
+                pair_j.append(jj)
+                pair_q.append(vals)
+                locs.append(locs[-1] + len(ii))
         self.bas_pair2shls = np.ascontiguousarray(
             np.vstack([np.concatenate(pair_i), np.concatenate(pair_j)]).astype(np.int32))
         self.bas_pairs_locs = np.asarray(locs, dtype=np.int32)
@@ -203,6 +209,18 @@ class _VHFOpt:
         out = np.zeros(3)
         mjk.check(mjk.load().mjk_last_stats(self.bp, mjk.ptr(out)), 'mjk_last_stats')
         return {'quartets_fp64': int(out[0]), 'quartets_fp32': int(out[1]), 'launches': int(out[2])}
+
+
+def _morton_keys(atm, env, bas, cell=2.0):
+    """Z-order key of each shell's centre on a grid of `cell` bohr."""
+    ptr = atm[bas[:, 0], 1]
+    xyz = np.stack([env[ptr], env[ptr + 1], env[ptr + 2]], axis=1)
+    g = np.floor((xyz - xyz.min(axis=0)) / cell).astype(np.int64)
+    key = np.zeros(len(g), dtype=np.int64)
+    for bit in range(16):
+        for d in range(3):
+            key |= ((g[:, d] >> bit) & 1) << (3 * bit + d)
+    return key
 
 
 def _split_l_ctr_groups(uniq_l_ctr, l_ctr_counts, group_size):

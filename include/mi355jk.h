@@ -55,8 +55,9 @@ int mjk_q_cond(const int32_t *atm, int natm, const int32_t *bas, int nbas,
  *   bas_pair2shls   int32[2][n_pairs]: bra shells then ket shells of every pair
  *   bas_pairs_locs  int32[ncptype + 1]: pair-class offsets into bas_pair2shls; all
  *                   pairs of a class share (l_i, nprim_i, l_j, nprim_j) with l_i >= l_j
- *   pair_q          double[n_pairs]: Schwarz factor of every pair, sorted descending
- *                   within each class
+ *   pair_q          double[n_pairs]: Schwarz factor of every pair.  Pairs may come in
+ *                   any order; ordering them by (i, j) within a class (shells sorted in
+ *                   space) lets the kernels reuse exchange partial sums across kets
  *   atm/bas/env     Cartesian, segmented basis (PySCF layout); env coefficients carry
  *                   the radial normalisation.
  */

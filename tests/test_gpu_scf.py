@@ -15,14 +15,16 @@ def _water_zmat(r=0.96, theta=104.5):
 
 
 def test_water_ccpvdz_literature():
-    """O-H 0.96 A, 104.5 deg, cc-pVDZ: E_nuc = 9.1681932964, E_RHF = -76.0266327 (Psi4 tutorial)."""
+    """O-H 0.96 A, 104.5 deg, cc-pVDZ: E_nuc = 9.1681932964 and E_RHF = -76.0266327 as quoted
+    in the Psi4 tutorial, whose SCF runs with density fitting by default; that fitting error
+    is of order 1e-5 Eh, so this anchor only pins the basis data and the SCF to 3e-5 Eh."""
     mol = Mole(_water_zmat(), basis='cc-pvdz')
     assert abs(mol.energy_nuc() - 9.168193296424) < 1e-6
     mf = RHF(mol, conv_tol=1e-10)
     e = mf.kernel()
     print(f'water cc-pVDZ E = {e:.10f} in {mf.cycles} cycles', flush=True)
     assert mf.converged
-    assert abs(e - (-76.0266327341)) < 1e-5
+    assert abs(e - (-76.0266327341)) < 3e-5
 
 
 def test_basis_ordering_of_energies():
