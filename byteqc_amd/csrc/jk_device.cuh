@@ -131,9 +131,34 @@ __device__ inline void rys_root32(double T, int r, float &x, float &w)
     w = s * b1w - b2w + cw[0];
 }
 
+// One-root rule in closed form: w = F0(T), x = F1(T)/F0(T), with
+// F0 = sqrt(pi/T) erf(sqrt T)/2, F1 = (F0 - exp(-T))/(2T), and the Taylor series
+// F_n = sum_k (-T)^k / (k! (2n + 2k + 1)) below T = 0.1 where the difference cancels.
+__device__ inline void rys_root1_closed(double T, double &x, double &w)
+{
+    double f0, f1;
+    if (T < 0.1) {
+        double t = 1.0, s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+            s0 += t / (2 * k + 1);
+            s1 += t / (2 * k + 3);
+            t *= -T / (k + 1);
+        }
+        f0 = s0;
+        f1 = s1;
+    } else {
+        const double st = sqrt(T);
+        f0 = 0.886226925452758014 * erf(st) / st;  // sqrt(pi)/2
+        f1 = (f0 - exp(-T)) * 0.5 / T;
+    }
+    w = f0;
+    x = f1 / f0;
+}
+
 template <typename R, int N>
 __device__ inline void rys_root(double T, int r, R &x, R &w);
-template <> __device__ inline void rys_root<double, 1>(double T, int r, double &x, double &w) { rys_root64<1>(T, r, x, w); }
+template <> __device__ inline void rys_root<double, 1>(double T, int r, double &x, double &w) { rys_root1_closed(T, x, w); }
 template <> __device__ inline void rys_root<double, 2>(double T, int r, double &x, double &w) { rys_root64<2>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 3>(double T, int r, double &x, double &w) { rys_root64<3>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 4>(double T, int r, double &x, double &w) { rys_root64<4>(T, r, x, w); }

@@ -30,6 +30,18 @@ __host__ __device__ inline long reg_tri_tiles(long bi, long nb_ket, long kc)
     return r * m * (m + 1) / 2 + (bi > B ? (bi - B) * nb_ket : 0);
 }
 
+// Inclusive segmented sum over lanes; seg_head is the first lane of this lane's segment
+// (segments are runs of consecutive lanes).
+__device__ inline double seg_scan(double v, int lane, int seg_head)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double u = __shfl_up(v, off, 64);
+        if (lane - off >= seg_head) v += u;
+    }
+    return v;
+}
+
 __device__ inline double wave_sum(double v)
 {
 #pragma unroll
@@ -91,6 +103,21 @@ jk_reg_kernel(const JKArgs a)
         if (!__syncthreads_or(live)) return;
     }
 
+    // lanes holding bra pairs with the same shell i form runs (pairs are in (i, j) order):
+    // K_il is summed over such a run before its atomic
+    const int lane = t & 63;
+    int seg_head, seg_tail;
+    {
+        const int key = pvalid ? ish : -1 - lane;
+        const int prev = __shfl_up(key, 1, 64), next = __shfl_down(key, 1, 64);
+        const bool head = lane == 0 || prev != key;
+        const bool tail = lane == 63 || next != key;
+        const unsigned long long hm = __builtin_amdgcn_ballot_w64(head);
+        // first head at or below this lane
+        const unsigned long long below = hm & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
+        seg_head = 63 - __builtin_clzll(below);
+        seg_tail = tail;
+    }
     const double A[3] = {a.bx[ish], a.by[ish], a.bz[ish]};
     const double AB[3] = {A[0] - a.bx[jsh], A[1] - a.by[jsh], A[2] - a.bz[jsh]};
     const int nao = a.nao;
@@ -239,8 +266,8 @@ jk_reg_kernel(const JKArgs a)
                 if ((t & 63) == 0) unsafeAtomicAdd(VJ + (size_t)(k0 + c / NFD) * nao + l0 + c % NFD, s);
             }
         }
-        if (dok && active) {
-            kdirty = true;
+        if (dok) {
+            kdirty |= active;
             const size_t ri = (size_t)i0 * nao, rj = (size_t)j0 * nao;
             double kil[NFA][NFD], kjl[NFB][NFD];
 #pragma unroll
@@ -273,11 +300,16 @@ jk_reg_kernel(const JKArgs a)
 #pragma unroll
             for (int x = 0; x < NFA; ++x)
 #pragma unroll
-                for (int d = 0; d < NFD; ++d) unsafeAtomicAdd(VK + ri + (size_t)x * nao + l0 + d, kil[x][d]);
+                for (int d = 0; d < NFD; ++d) {
+                    const double v = seg_scan(kil[x][d], lane, seg_head);
+                    if (seg_tail && pvalid && v != 0.0) unsafeAtomicAdd(VK + ri + (size_t)x * nao + l0 + d, v);
+                }
+            if (active) {
 #pragma unroll
-            for (int x = 0; x < NFB; ++x)
+                for (int x = 0; x < NFB; ++x)
 #pragma unroll
-                for (int d = 0; d < NFD; ++d) unsafeAtomicAdd(VK + rj + (size_t)x * nao + l0 + d, kjl[x][d]);
+                    for (int d = 0; d < NFD; ++d) unsafeAtomicAdd(VK + rj + (size_t)x * nao + l0 + d, kjl[x][d]);
+            }
         }
     }
     // ---- final flushes
