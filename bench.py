@@ -110,12 +110,14 @@ def cpu_baseline(vhfopt, dm_cart_host, target_s):
 
 
 def run_scf(mol, precision_cutoff=0.0):
+    """RHF from the SAD guess to |dE| < 1e-8 Eh; time from the guess to convergence."""
     from byteqc_amd.scf.rhf import RHF
     mf = RHF(mol, conv_tol=1e-8, precision_cutoff=precision_cutoff)
     mf.build()
     e = mf.kernel()
     return {'e_tot': e, 'converged': mf.converged, 'cycles': mf.cycles,
-            'time_s': mf.timings['total'], 'jk_s': mf.timings['jk'],
+            'time_s': mf.timings['scf'], 'guess_s': mf.timings['guess'], 'jk_s': mf.timings['jk'],
+            'setup_s': mf.timings['vhfopt'] + mf.timings['int1e'],
             'precision_cutoff': precision_cutoff}
 
 
@@ -203,15 +205,27 @@ def main(argv=None):
         prof = per_class_profile(vhfopt, dmc, vj, vk, screen, torch, lib, mjk)
         tot_ms = sum(p['ms'] for p in prof)
         tot_fl = sum(p['flops'] for p in prof)
-        dom = max(prof, key=lambda p: p['ms'])
+        # one kernel symbol per angular-momentum class: aggregate its launches (one per
+        # primitive-count combination) the way rocprofv3 --stats does
+        sym = {}
+        for p in prof:
+            a = sym.setdefault(p['class'], {'ms': 0.0, 'flops': 0.0, 'launches': 0, 'quartets': 0})
+            a['ms'] += p['ms']
+            a['flops'] += p['flops']
+            a['launches'] += 1
+            a['quartets'] += p['quartets']
+        dname, dom = max(sym.items(), key=lambda kv: kv[1]['ms'])
         ach = dom['flops'] / (dom['ms'] * 1e-3) / 1e12
+        from byteqc_amd.perf.flops import kernel_symbol
         result['roofline'] = {
             'bound': 'mfma', 'achieved': ach, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
             'frac': ach / FP64_PEAK_TFLOPS, 'traffic': None,
-            'kernel': f'jk_kernel<double,{dom["class"]}> nprim={dom["nprim"]}',
-            'quartets': dom['quartets'], 'launch_ms': dom['ms'],
-            'note': 'FP64 vector-ALU bound (Rys recursion + digestion); MI355X FP64 vector and '
-                    'matrix peaks are both 78.6 TFLOP/s',
+            'kernel': kernel_symbol(dname),
+            'launches': dom['launches'], 'avg_launch_ms': dom['ms'] / dom['launches'],
+            'quartets': dom['quartets'], 'flops_per_build': dom['flops'],
+            'note': 'FP64 vector-ALU roof (Rys recursion + digestion); MI355X FP64 vector and '
+                    'matrix peaks are both 78.6 TFLOP/s.  achieved = algorithmic FLOPs of all '
+                    'launches of this kernel symbol in one build / their summed HIP-event time',
         }
         result['build_tflops'] = tot_fl / (tot_ms * 1e-3) / 1e12
         if args.classes_out:
@@ -222,14 +236,21 @@ def main(argv=None):
         result['top_classes'] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()
                                   if k != 'cp'} for p in top]
         log(f'per-class sum {tot_ms:.1f} ms, {result["build_tflops"]:.2f} TFLOP/s overall; '
-            f'dominant {dom["class"]} {dom["ms"]:.2f} ms {ach:.2f} TFLOP/s')
+            f'dominant {dname} {dom["ms"]:.2f} ms over {dom["launches"]} launches, {ach:.2f} TFLOP/s')
     if not args.no_cpu:
         dmh = dmc.reshape(nao, nao).cpu().numpy()
         result['cpu_baseline'] = cpu_baseline(vhfopt, dmh, args.cpu_seconds)
         log(f'cpu baseline {result["cpu_baseline"]}')
     if args.scf:
-        result['scf'] = run_scf(mol)
-        log(f'scf {result["scf"]}')
+        vhfopt.clear()
+        torch.cuda.empty_cache()
+        result['scf'] = {'fp64_sad': run_scf(mol)}
+        log(f'scf fp64 {result["scf"]["fp64_sad"]}')
+        result['scf']['mixed_sad'] = run_scf(mol, precision_cutoff=1e-7)
+        r = result['scf']
+        r['mixed_delta_e'] = r['mixed_sad']['e_tot'] - r['fp64_sad']['e_tot']
+        r['speedup'] = r['fp64_sad']['time_s'] / r['mixed_sad']['time_s']
+        log(f'scf mixed {r["mixed_sad"]}  dE={r["mixed_delta_e"]:.2e} speedup={r["speedup"]:.2f}')
     print(json.dumps(result), flush=True)
     return 0
 

@@ -41,3 +41,16 @@ def class_flops(la, lb, lc, ld, nprim_quartet=1, n_dm=1, with_j=True, with_k=Tru
 def class_label(la, lb, lc, ld):
     s = 'spdfg'
     return f'({s[la]}{s[lb]}|{s[lc]}{s[ld]})'
+
+
+def kernel_symbol(label, fp32=False):
+    """Kernel template instance that runs a class label such as '(dp|ps)'."""
+    import re
+    la, lb, lc, ld = ['spdfg'.index(x) for x in re.findall('[spdfg]', label)]
+    nc = ncart
+    gs = (la + 1) * (lb + 1) * (lc + 1) * (ld + 1)
+    regs = 3 * gs + nc(la) * nc(lb) * nc(lc) * nc(ld) + nc(la) * nc(lb) + (nc(la) + nc(lb)) * nc(lc)
+    name = 'jk_reg_kernel' if regs <= 150 else 'jk_grp_kernel'
+    if (la, lb, lc, ld) in ((3, 3, 3, 2), (3, 3, 3, 3)):
+        name = 'jk_kernel'  # the block-LDS kernel keeps the two largest classes
+    return f'mjk::{name}<{"float" if fp32 else "double"}, {la}, {lb}, {lc}, {ld}>'
