@@ -158,7 +158,11 @@ __device__ inline void rys_root1_closed(double T, double &x, double &w)
 
 template <typename R, int N>
 __device__ inline void rys_root(double T, int r, R &x, R &w);
+#ifdef MJK_TABLE_ROOT1
+template <> __device__ inline void rys_root<double, 1>(double T, int r, double &x, double &w) { rys_root64<1>(T, r, x, w); }
+#else
 template <> __device__ inline void rys_root<double, 1>(double T, int r, double &x, double &w) { rys_root1_closed(T, x, w); }
+#endif
 template <> __device__ inline void rys_root<double, 2>(double T, int r, double &x, double &w) { rys_root64<2>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 3>(double T, int r, double &x, double &w) { rys_root64<3>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 4>(double T, int r, double &x, double &w) { rys_root64<4>(T, r, x, w); }

@@ -25,6 +25,7 @@ from ..gto.cart2sph import cart2sph_block
 from ..gto.mole import ANG_OF, NPRIM_OF, NCTR_OF, PTR_EXP, PTR_COEFF, gto_norm
 
 LMAX_ON_GPU = mjk.MJK_LMAX
+PAIR_BUCKETS_PER_DECADE = 1
 
 
 def _torch():
@@ -91,10 +92,11 @@ class _VHFOpt:
     build() follows cuobc/scf/hf.py:288-408: shells are grouped by (l, nprim); for every
     group pair (p >= q) the shell pairs with Schwarz factor above `cutoff` are kept,
     lower triangle only on diagonal groups; the pair data goes to the device with e12
-    halved on same-shell pairs.  Unlike the reference (which sorts pairs by descending
-    factor for its log-q bins), pairs stay in (i, j) order and shells inside a group are
-    ordered along a space-filling curve: the kernels screen each quartet themselves and
-    use the shared shells of neighbouring pairs to cut exchange atomics.
+    halved on same-shell pairs.  Where the reference sorts pairs by descending factor
+    for its log-q bins, here pairs are ordered by decade of the factor and then by (i, j),
+    with shells inside a group ordered along a space-filling curve: tiles of pairs from
+    one decade are screened as a whole, and neighbouring pairs share shells, which the
+    kernels use to cut exchange atomics.
     """
 
     def __init__(self, mol, intor='int2e', prescreen='CVHFnoscreen',
@@ -156,12 +158,16 @@ class _VHFOpt:
                 if qg == p and not diag_block_with_triu:
                     mask &= ii >= jj
                 ii, jj, vals = ii[mask], jj[mask], vals[mask]
-                # (i, j) order: consecutive kets share shell k, which the kernels use to
-                # keep exchange partial sums in registers across kets
-                pair_i.append(ii)
-                pair_j.append(jj)
-                pair_q.append(vals)
-                locs.append(locs[-1] + len(ii))
+                # decades of the Schwarz factor, strongest first, then (i, j): a tile of
+                # pairs from one decade is screened as a whole, and inside a decade
+                # consecutive pairs share shell i, which the kernels use to keep exchange
+                # partial sums in registers and to reduce across lanes
+                dec = np.floor(np.log10(np.maximum(vals, 1e-300)) * PAIR_BUCKETS_PER_DECADE)
+                idx = np.lexsort((jj, ii, -dec))
+                pair_i.append(ii[idx])
+                pair_j.append(jj[idx])
+                pair_q.append(vals[idx])
+                locs.append(locs[-1] + len(idx))
         self.bas_pair2shls = np.ascontiguousarray(
             np.vstack([np.concatenate(pair_i), np.concatenate(pair_j)]).astype(np.int32))
         self.bas_pairs_locs = np.asarray(locs, dtype=np.int32)
