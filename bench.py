@@ -59,7 +59,7 @@ def per_class_profile(vhfopt, dmc, vj, vk, screen, torch, lib, mjk):
                                    ctypes.byref(screen), ctypes.c_void_p(stream.cuda_stream)), 'warm')
     torch.cuda.synchronize()
     out = []
-    prev = np.zeros(3)
+    prev = np.zeros(4)
     mjk.check(lib.mjk_last_stats(vhfopt.bp, mjk.ptr(prev)), 'stats')
     for a in range(vhfopt.ncptype):
         for b in range(a + 1):
@@ -73,15 +73,17 @@ def per_class_profile(vhfopt, dmc, vj, vk, screen, torch, lib, mjk):
                       'mjk_build_jk')
             e1.record(stream)
             e1.synchronize()
-            cur = np.zeros(3)
+            cur = np.zeros(4)
             mjk.check(lib.mjk_last_stats(vhfopt.bp, mjk.ptr(cur)), 'stats')
             nq = cur[0] - prev[0] + cur[1] - prev[1]
+            nslots = cur[3] - prev[3]
             prev = cur
             la, lb, pa = info[a]
             lc, ld, pb = info[b]
             fl = nq * class_flops(la, lb, lc, ld, pa * pb)
             out.append({'class': class_label(la, lb, lc, ld), 'cp': (a, b), 'nprim': pa * pb,
-                        'quartets': int(nq), 'ms': e0.elapsed_time(e1), 'flops': fl})
+                        'quartets': int(nq), 'slots': int(nslots), 'ms': e0.elapsed_time(e1),
+                        'flops': fl})
     return out
 
 
@@ -187,7 +189,8 @@ def main(argv=None):
     t1 = time.perf_counter()
     sec = (t1 - t0) / args.steps
     st = vhfopt.stats()
-    log(f'J/K build: {sec * 1e3:.1f} ms/step, quartets fp64={st["quartets_fp64"]} fp32={st["quartets_fp32"]}')
+    log(f'J/K build: {sec * 1e3:.1f} ms/step, quartets fp64={st["quartets_fp64"]} fp32={st["quartets_fp32"]} '
+        f'slots={st["slots"]}')
 
     result = {
         'metric': 'J/K build s/iter (plain FP64)',

@@ -52,6 +52,18 @@ struct PairSet {
     const int *ish, *jsh;
     const double *q;
     const double *a, *e, *px, *py, *pz;  // [npairs * nprim]
+    // per pair: AO offsets of both shells, centre of shell i and R_i - R_j, so that a
+    // pair's geometry is one level of independent loads
+    const int *aoi, *aoj;
+    const double *cx, *cy, *cz, *abx, *aby, *abz;
+    // packed records used by the register and group kernels (three base pointers keep
+    // the kernel arguments in scalar registers):
+    //   pint[p] = {ish, jsh, ao_i, ao_j}
+    //   pdbl[8p + 0..6] = {q, x_i, y_i, z_i, (R_i - R_j)_x, _y, _z}
+    //   prim[6 (p nprim + k) + 0..5] = {a, e, P_x, P_y, P_z, 1/a}
+    const int4 *pint;
+    const double *pdbl;
+    const double *prim;
     int npairs, nprim;
 };
 
@@ -63,6 +75,7 @@ struct JKArgs {
     double *vj, *vk;
     const double *dm_cond;  // nbas x nbas shell-block max |D| or null
     unsigned long long *counter;  // quartets computed (optional)
+    unsigned long long *slots;    // lane slots of executed ket iterations (optional)
     double tol;             // keep quartets with bound > tol
     double lo, hi;          // this launch takes lo <= bound < hi
     int nbas, nao, n_dm;
@@ -156,20 +169,47 @@ __device__ inline void rys_root1_closed(double T, double &x, double &w)
     x = f1 / f0;
 }
 
+// One-root rule from the Boys grid: F_n(T_i - d) = sum_k F_(n+k)(T_i) d^k / k!, ten terms,
+// |d| <= h/2 = 0.05 (truncation below 1e-17); above the grid F0 = sqrt(pi/T)/2 and
+// F1/F0 = 1/(2T) (exp(-T) < 5e-18 there).
+__device__ inline void rys_root1_grid(double T, double &x, double &w)
+{
+    if (T >= RYS_BOYS_TMAX) {
+        w = 0.886226925452758014 / sqrt(T);
+        x = 0.5 / T;
+        return;
+    }
+    constexpr double inv_fact[10] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
+                                     1.0 / 5040, 1.0 / 40320, 1.0 / 362880};
+    const int i = (int)(T * (1.0 / RYS_BOYS_H) + 0.5);
+    const double d = i * RYS_BOYS_H - T;
+    const double *g = rys_boys_grid + i * (RYS_BOYS_K + 1);
+    double f0 = g[9] * inv_fact[9], f1 = g[10] * inv_fact[9];
+#pragma unroll
+    for (int k = 8; k >= 0; --k) {
+        f0 = f0 * d + g[k] * inv_fact[k];
+        f1 = f1 * d + g[k + 1] * inv_fact[k];
+    }
+    w = f0;
+    x = f1 / f0;
+}
+
 template <typename R, int N>
 __device__ inline void rys_root(double T, int r, R &x, R &w);
-#ifdef MJK_TABLE_ROOT1
-template <> __device__ inline void rys_root<double, 1>(double T, int r, double &x, double &w) { rys_root64<1>(T, r, x, w); }
-#else
-template <> __device__ inline void rys_root<double, 1>(double T, int r, double &x, double &w) { rys_root1_closed(T, x, w); }
-#endif
+template <> __device__ inline void rys_root<double, 1>(double T, int r, double &x, double &w) { rys_root1_grid(T, x, w); }
 template <> __device__ inline void rys_root<double, 2>(double T, int r, double &x, double &w) { rys_root64<2>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 3>(double T, int r, double &x, double &w) { rys_root64<3>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 4>(double T, int r, double &x, double &w) { rys_root64<4>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 5>(double T, int r, double &x, double &w) { rys_root64<5>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 6>(double T, int r, double &x, double &w) { rys_root64<6>(T, r, x, w); }
 template <> __device__ inline void rys_root<double, 7>(double T, int r, double &x, double &w) { rys_root64<7>(T, r, x, w); }
-template <> __device__ inline void rys_root<float, 1>(double T, int r, float &x, float &w) { rys_root32<1>(T, r, x, w); }
+template <> __device__ inline void rys_root<float, 1>(double T, int r, float &x, float &w)
+{
+    double xd, wd;
+    rys_root1_grid(T, xd, wd);
+    x = (float)xd;
+    w = (float)wd;
+}
 template <> __device__ inline void rys_root<float, 2>(double T, int r, float &x, float &w) { rys_root32<2>(T, r, x, w); }
 template <> __device__ inline void rys_root<float, 3>(double T, int r, float &x, float &w) { rys_root32<3>(T, r, x, w); }
 template <> __device__ inline void rys_root<float, 4>(double T, int r, float &x, float &w) { rys_root32<4>(T, r, x, w); }

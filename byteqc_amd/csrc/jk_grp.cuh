@@ -82,17 +82,15 @@ jk_grp_kernel(const JKArgs a)
     if (q_end > a.ket.npairs) q_end = a.ket.npairs;
 
     const bool pvalid = lane_ok && p < a.bra.npairs;
-    int ish = 0, jsh = 0;
-    double qp = 0.0;
-    if (pvalid) {
-        ish = a.bra.ish[p];
-        jsh = a.bra.jsh[p];
-        qp = a.bra.q[p];
-    }
+    const int pp = pvalid ? p : 0;
+    const int4 b4 = a.bra.pint[pp];
+    const double *bd = a.bra.pdbl + 8 * pp;
+    const int ish = b4.x, jsh = b4.y;
+    const double qp = pvalid ? bd[0] : 0.0;
     const double dmax = a.dm_max ? *a.dm_max : 1.0;
     {
         double qk = 0.0;
-        for (int q = q_begin + lane; q < q_end; q += 64) qk = fmax(qk, a.ket.q[q]);
+        for (int q = q_begin + lane; q < q_end; q += 64) qk = fmax(qk, a.ket.pdbl[8 * q]);
         double qb = qp;
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
@@ -101,10 +99,10 @@ jk_grp_kernel(const JKArgs a)
         }
         if (!(qb * qk * dmax > a.tol && qb * qk * dmax >= a.lo)) return;  // wave-uniform
     }
-    const double A[3] = {a.bx[ish], a.by[ish], a.bz[ish]};
-    const double AB[3] = {A[0] - a.bx[jsh], A[1] - a.by[jsh], A[2] - a.bz[jsh]};
+    const double A[3] = {bd[1], bd[2], bd[3]};
+    const double AB[3] = {bd[4], bd[5], bd[6]};
     const int nao = a.nao;
-    const int i0 = a.ao_loc[ish], j0 = a.ao_loc[jsh];
+    const int i0 = b4.z, j0 = b4.w;
     const double *D = a.dm;
     double *VJ = a.vj, *VK = a.vk;
     const bool doj = VJ != nullptr, dok = VK != nullptr;
@@ -130,14 +128,17 @@ jk_grp_kernel(const JKArgs a)
     for (int x = sub; x < C::KBUF; x += S)
         if (lane_ok) kbuf[gg][x] = 0.0;
     int cur_k = -1, cur_k0 = 0;
-    unsigned long long nquart = 0;
+    unsigned long long nquart = 0, nslots = 0;
     const int npb = a.bra.nprim, npk = a.ket.nprim;
     __syncthreads();
 
     for (int q = q_begin; q < q_end; ++q) {
-        const int ksh = __builtin_amdgcn_readfirstlane(a.ket.ish[q]);
-        const int lsh = __builtin_amdgcn_readfirstlane(a.ket.jsh[q]);
-        const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+        const int4 k4 = a.ket.pint[q];
+        const int ksh = __builtin_amdgcn_readfirstlane(k4.x);
+        const int lsh = __builtin_amdgcn_readfirstlane(k4.y);
+        const int k0 = __builtin_amdgcn_readfirstlane(k4.z);
+        const int l0 = __builtin_amdgcn_readfirstlane(k4.w);
+        const double *kd = a.ket.pdbl + 8 * q;
         if (ksh != cur_k) {
             if (dok && cur_k >= 0 && pvalid) {
                 for (int x = sub; x < C::KIL; x += S) {
@@ -155,7 +156,7 @@ jk_grp_kernel(const JKArgs a)
         }
         bool active = pvalid && (!a.same || q <= p);
         if (active) {
-            double bound = qp * a.ket.q[q];
+            double bound = qp * kd[0];
             if (a.dm_cond) {
                 const double *dc = a.dm_cond;
                 const int nb = a.nbas;
@@ -168,37 +169,37 @@ jk_grp_kernel(const JKArgs a)
         }
         if (!__builtin_amdgcn_ballot_w64(active)) continue;  // wave-uniform
         if (active && sub == 0) ++nquart;
+        ++nslots;
         R acc[RPT][NFCD];
 #pragma unroll
         for (int k = 0; k < RPT; ++k)
 #pragma unroll
             for (int c = 0; c < NFCD; ++c) acc[k][c] = (R)0;
-        const double Cc[3] = {a.bx[ksh], a.by[ksh], a.bz[ksh]};
-        const double CD[3] = {Cc[0] - a.bx[lsh], Cc[1] - a.by[lsh], Cc[2] - a.bz[lsh]};
+        const double Cc[3] = {kd[1], kd[2], kd[3]};
+        const double CD[3] = {kd[4], kd[5], kd[6]};
         const double norm = (a.same && p == q) ? 0.5 * PI52X2 : PI52X2;
         for (int ip = 0; ip < npb; ++ip) {
             for (int kq = 0; kq < npk; ++kq) {
-                const int pi_ = p * npb + ip, qi_ = q * npk + kq;
-                const double pref = active ? norm * a.bra.e[pi_] * a.ket.e[qi_] /
-                                                 (a.bra.a[pi_] * a.ket.a[qi_] * sqrt(a.bra.a[pi_] + a.ket.a[qi_]))
-                                           : 0.0;
+                const double *bp_ = a.bra.prim + 6 * (pp * npb + ip);
+                const double *kp_ = a.ket.prim + 6 * (q * npk + kq);
+                const double aij = bp_[0], akl = kp_[0], iaij = bp_[5], iakl = kp_[5];
+                const double iaijkl = 1.0 / (aij + akl);
+                const double pref = active ? norm * bp_[1] * kp_[1] * iaij * iakl * sqrt(iaijkl) : 0.0;
                 const bool pactive = active && fabs(pref) >= a.prim_tol;
                 if (!__builtin_amdgcn_ballot_w64(pactive)) continue;  // wave-uniform: barriers stay paired
                 if (pactive) {
-                    const double aij = a.bra.a[pi_], akl = a.ket.a[qi_];
-                    const double P[3] = {a.bra.px[pi_], a.bra.py[pi_], a.bra.pz[pi_]};
-                    const double Q[3] = {a.ket.px[qi_], a.ket.py[qi_], a.ket.pz[qi_]};
+                    const double P[3] = {bp_[2], bp_[3], bp_[4]};
+                    const double Q[3] = {kp_[2], kp_[3], kp_[4]};
                     const double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
-                    const double aijkl = aij + akl;
-                    const double T = aij * akl / aijkl * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-                    const double inv2 = 0.5 / aijkl;
+                    const double T = aij * akl * iaijkl * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+                    const double inv2 = 0.5 * iaijkl;
                     for (int it = sub; it < 3 * NR; it += S) {
                         const int r = it / 3, d = it % 3;
                         R x, w;
                         rys_root<R, NR>(T, r, x, w);
                         const R b00 = x * (R)inv2;
-                        const R b10 = ((R)0.5 - b00 * (R)akl) / (R)aij;
-                        const R b01 = ((R)0.5 - b00 * (R)aij) / (R)akl;
+                        const R b10 = ((R)0.5 - b00 * (R)akl) * (R)iaij;
+                        const R b01 = ((R)0.5 - b00 * (R)aij) * (R)iakl;
                         const R tb = (R)2 * b00;
                         const double Pd = d == 0 ? P[0] : (d == 1 ? P[1] : P[2]);
                         const double Qd = d == 0 ? Q[0] : (d == 1 ? Q[1] : Q[2]);
@@ -331,6 +332,7 @@ jk_grp_kernel(const JKArgs a)
     if (a.counter) {
         const double cnt = wave_sum((double)nquart);
         if (lane == 0 && cnt > 0) atomicAdd(a.counter, (unsigned long long)cnt);
+        if (lane == 0 && a.slots) atomicAdd(a.slots, (unsigned long long)W * nslots);
     }
 }
 

@@ -8,8 +8,11 @@
 //   * J_ij accumulates per lane over the whole chunk and is flushed once;
 //   * J_kl is uniform per ket: a wave-wide sum, then one atomic per element per wave;
 //   * K_ik and K_jk accumulate per lane while consecutive kets share shell k (ket pairs
-//     are ordered by (k, l)) and are flushed when k changes;
-//   * K_il and K_jl are flushed per quartet.
+//     are ordered by (k, l) inside a Schwarz decade) and are flushed when k changes;
+//   * K_il is summed over the run of lanes whose bra pairs share shell i, then written;
+//   * K_jl is written per quartet.
+// Every D element a quartet needs is loaded at the top of its iteration, so the load
+// latency hides behind the integral evaluation.
 // The six updates are those of cuobc/lib/gvhf/contract_jk.cu:219-316.
 #pragma once
 #include "jk_device.cuh"
@@ -17,7 +20,7 @@
 namespace mjk {
 
 constexpr int REG_BRA = 256;  // bra pairs (lanes) per tile
-constexpr int REG_KC = 128;   // ket pairs per tile
+constexpr int REG_KC = 128;   // longest ket chunk per tile
 
 // Tiles of a same-class launch with ket chunks of kc pairs (kc divides REG_BRA): bra block
 // b needs ket chunks 0 .. min(nb_ket, (b+1) r) - 1, r = REG_BRA / kc.  Returns the number of
@@ -75,24 +78,23 @@ jk_reg_kernel(const JKArgs a)
         ki = (int)(tile - reg_tri_tiles(lo, a.nb_ket, a.kc));
     }
     const int t = threadIdx.x;
+    const int lane = t & 63;
     const int p = bi * REG_BRA + t;
     const int q_begin = ki * a.kc;
     int q_end = q_begin + a.kc;
     if (q_end > a.ket.npairs) q_end = a.ket.npairs;
 
     const bool pvalid = p < a.bra.npairs;
-    int ish = 0, jsh = 0;
-    double qp = 0.0;
-    if (pvalid) {
-        ish = a.bra.ish[p];
-        jsh = a.bra.jsh[p];
-        qp = a.bra.q[p];
-    }
+    const int pp = pvalid ? p : 0;
+    const int4 b4 = a.bra.pint[pp];
+    const double *bd = a.bra.pdbl + 8 * pp;
+    const int ish = b4.x, jsh = b4.y;
+    const double qp = pvalid ? bd[0] : 0.0;
     const double dmax = a.dm_max ? *a.dm_max : 1.0;
     // cheap tile test: the largest bound any quartet of the tile can reach
     {
         double qk = 0.0;
-        for (int q = q_begin + (t & 63); q < q_end; q += 64) qk = fmax(qk, a.ket.q[q]);
+        for (int q = q_begin + lane; q < q_end; q += 64) qk = fmax(qk, a.ket.pdbl[8 * q]);
         double qb = qp;
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
@@ -102,10 +104,7 @@ jk_reg_kernel(const JKArgs a)
         const int live = qb * qk * dmax > a.tol && qb * qk * dmax >= a.lo;
         if (!__syncthreads_or(live)) return;
     }
-
-    // lanes holding bra pairs with the same shell i form runs (pairs are in (i, j) order):
-    // K_il is summed over such a run before its atomic
-    const int lane = t & 63;
+    // lanes holding bra pairs with the same shell i form runs: K_il is summed over a run
     int seg_head, seg_tail;
     {
         const int key = pvalid ? ish : -1 - lane;
@@ -113,19 +112,22 @@ jk_reg_kernel(const JKArgs a)
         const bool head = lane == 0 || prev != key;
         const bool tail = lane == 63 || next != key;
         const unsigned long long hm = __builtin_amdgcn_ballot_w64(head);
-        // first head at or below this lane
         const unsigned long long below = hm & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
         seg_head = 63 - __builtin_clzll(below);
         seg_tail = tail;
     }
-    const double A[3] = {a.bx[ish], a.by[ish], a.bz[ish]};
-    const double AB[3] = {A[0] - a.bx[jsh], A[1] - a.by[jsh], A[2] - a.bz[jsh]};
+    const double A[3] = {bd[1], bd[2], bd[3]};
+    const double AB[3] = {bd[4], bd[5], bd[6]};
     const int nao = a.nao;
-    const int i0 = a.ao_loc[ish], j0 = a.ao_loc[jsh];
-    const double *D = a.dm;
+    const int i0 = b4.z, j0 = b4.w;
+    const double *__restrict__ D = a.dm;
     double *VJ = a.vj, *VK = a.vk;
     const bool doj = VJ != nullptr, dok = VK != nullptr;
+    const int ri = i0 * nao, rj = j0 * nao;  // row offsets (nao^2 < 2^31)
 
+    double dij[NFAB];
+#pragma unroll
+    for (int r = 0; r < NFAB; ++r) dij[r] = doj ? D[ri + (r / NFB) * nao + j0 + r % NFB] : 0.0;
     double jij[NFAB];
     double kik[NFA][NFC], kjk[NFB][NFC];
 #pragma unroll
@@ -140,27 +142,52 @@ jk_reg_kernel(const JKArgs a)
         for (int c = 0; c < NFC; ++c) kjk[x][c] = 0.0;
     int cur_k = -1, cur_k0 = 0;
     bool kdirty = false;
-    unsigned long long nquart = 0;
+    unsigned long long nquart = 0, nslots = 0;
     const int npb = a.bra.nprim, npk = a.ket.nprim;
 
     for (int q = q_begin; q < q_end; ++q) {
-        const int ksh = __builtin_amdgcn_readfirstlane(a.ket.ish[q]);
-        const int lsh = __builtin_amdgcn_readfirstlane(a.ket.jsh[q]);
-        const int k0 = a.ao_loc[ksh], l0 = a.ao_loc[lsh];
+        // ---- ket q: all uniform, one level of loads
+        const int4 k4 = a.ket.pint[q];
+        const int ksh = __builtin_amdgcn_readfirstlane(k4.x);
+        const int lsh = __builtin_amdgcn_readfirstlane(k4.y);
+        const int k0 = __builtin_amdgcn_readfirstlane(k4.z);
+        const int l0 = __builtin_amdgcn_readfirstlane(k4.w);
+        const double *kd = a.ket.pdbl + 8 * q;
+        const double qq = kd[0];
+        const double Cc[3] = {kd[1], kd[2], kd[3]};
+        const double CD[3] = {kd[4], kd[5], kd[6]};
+        // ---- D elements of this quartet, issued before the integrals
+        double dkl[NFCD], dik[NFA][NFC], djk[NFB][NFC], dil[NFA][NFD], djl[NFB][NFD];
+#pragma unroll
+        for (int c = 0; c < NFCD; ++c) dkl[c] = doj ? D[(k0 + c / NFD) * nao + l0 + c % NFD] : 0.0;
+#pragma unroll
+        for (int x = 0; x < NFA; ++x) {
+#pragma unroll
+            for (int c = 0; c < NFC; ++c) dik[x][c] = dok ? D[ri + x * nao + k0 + c] : 0.0;
+#pragma unroll
+            for (int d = 0; d < NFD; ++d) dil[x][d] = dok ? D[ri + x * nao + l0 + d] : 0.0;
+        }
+#pragma unroll
+        for (int x = 0; x < NFB; ++x) {
+#pragma unroll
+            for (int c = 0; c < NFC; ++c) djk[x][c] = dok ? D[rj + x * nao + k0 + c] : 0.0;
+#pragma unroll
+            for (int d = 0; d < NFD; ++d) djl[x][d] = dok ? D[rj + x * nao + l0 + d] : 0.0;
+        }
         if (ksh != cur_k) {
             if (dok && kdirty) {
 #pragma unroll
                 for (int x = 0; x < NFA; ++x)
 #pragma unroll
                     for (int c = 0; c < NFC; ++c) {
-                        unsafeAtomicAdd(VK + (size_t)(i0 + x) * nao + cur_k0 + c, kik[x][c]);
+                        unsafeAtomicAdd(VK + ri + x * nao + cur_k0 + c, kik[x][c]);
                         kik[x][c] = 0.0;
                     }
 #pragma unroll
                 for (int x = 0; x < NFB; ++x)
 #pragma unroll
                     for (int c = 0; c < NFC; ++c) {
-                        unsafeAtomicAdd(VK + (size_t)(j0 + x) * nao + cur_k0 + c, kjk[x][c]);
+                        unsafeAtomicAdd(VK + rj + x * nao + cur_k0 + c, kjk[x][c]);
                         kjk[x][c] = 0.0;
                     }
             }
@@ -170,7 +197,7 @@ jk_reg_kernel(const JKArgs a)
         }
         bool active = pvalid && (!a.same || q <= p);
         if (active) {
-            double bound = qp * a.ket.q[q];
+            double bound = qp * qq;
             if (a.dm_cond) {
                 const double *dc = a.dm_cond;
                 const int nb = a.nbas;
@@ -182,6 +209,7 @@ jk_reg_kernel(const JKArgs a)
             active = bound > a.tol && bound >= a.lo && bound < a.hi;
         }
         if (!__builtin_amdgcn_ballot_w64(active)) continue;  // wave-uniform skip
+        ++nslots;
         R acc[NFAB][NFCD];
 #pragma unroll
         for (int r = 0; r < NFAB; ++r)
@@ -189,30 +217,28 @@ jk_reg_kernel(const JKArgs a)
             for (int c = 0; c < NFCD; ++c) acc[r][c] = (R)0;
         if (active) {
             ++nquart;
-            const double Cc[3] = {a.bx[ksh], a.by[ksh], a.bz[ksh]};
-            const double CD[3] = {Cc[0] - a.bx[lsh], Cc[1] - a.by[lsh], Cc[2] - a.bz[lsh]};
             const double norm = (a.same && p == q) ? 0.5 * PI52X2 : PI52X2;
             for (int ip = 0; ip < npb; ++ip) {
-                const int pi_ = p * npb + ip;
-                const double aij = a.bra.a[pi_], eij = a.bra.e[pi_];
-                const double P[3] = {a.bra.px[pi_], a.bra.py[pi_], a.bra.pz[pi_]};
+                const double *bp_ = a.bra.prim + 6 * (p * npb + ip);
+                const double aij = bp_[0], eij = bp_[1], iaij = bp_[5];
+                const double P[3] = {bp_[2], bp_[3], bp_[4]};
                 for (int kq = 0; kq < npk; ++kq) {
-                    const int qi_ = q * npk + kq;
-                    const double akl = a.ket.a[qi_], ekl = a.ket.e[qi_];
-                    const double Q[3] = {a.ket.px[qi_], a.ket.py[qi_], a.ket.pz[qi_]};
+                    const double *kp_ = a.ket.prim + 6 * (q * npk + kq);
+                    const double akl = kp_[0], ekl = kp_[1], iakl = kp_[5];
+                    const double Q[3] = {kp_[2], kp_[3], kp_[4]};
                     const double PQ[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
-                    const double aijkl = aij + akl;
-                    const double T = aij * akl / aijkl * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-                    const double pref = norm * eij * ekl / (aij * akl * sqrt(aijkl));
+                    const double iaijkl = 1.0 / (aij + akl);
+                    const double T = aij * akl * iaijkl * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+                    const double pref = norm * eij * ekl * iaij * iakl * sqrt(iaijkl);
                     if (fabs(pref) < a.prim_tol) continue;
-                    const double inv2 = 0.5 / aijkl;
+                    const double inv2 = 0.5 * iaijkl;
 #pragma unroll 1
                     for (int r = 0; r < NR; ++r) {
                         R x, w;
                         rys_root<R, NR>(T, r, x, w);
                         const R b00 = x * (R)inv2;
-                        const R b10 = ((R)0.5 - b00 * (R)akl) / (R)aij;
-                        const R b01 = ((R)0.5 - b00 * (R)aij) / (R)akl;
+                        const R b10 = ((R)0.5 - b00 * (R)akl) * (R)iaij;
+                        const R b01 = ((R)0.5 - b00 * (R)aij) * (R)iakl;
                         const R tb = (R)2 * b00;
                         R gx[GS], gy[GS], gz[GS];
                         rys2d<R, LA, LB, LC, LD>((R)1, (R)(P[0] - A[0]) - tb * (R)akl * (R)PQ[0],
@@ -248,27 +274,21 @@ jk_reg_kernel(const JKArgs a)
             double jkl[NFCD];
 #pragma unroll
             for (int c = 0; c < NFCD; ++c) jkl[c] = 0.0;
-            if (active) {
 #pragma unroll
-                for (int r = 0; r < NFAB; ++r) {
-                    const double dij = D[(size_t)(i0 + r / NFB) * nao + j0 + r % NFB];
+            for (int r = 0; r < NFAB; ++r)
 #pragma unroll
-                    for (int c = 0; c < NFCD; ++c) {
-                        const double dkl = D[(size_t)(k0 + c / NFD) * nao + l0 + c % NFD];
-                        jij[r] += (double)acc[r][c] * dkl;
-                        jkl[c] += (double)acc[r][c] * dij;
-                    }
+                for (int c = 0; c < NFCD; ++c) {
+                    jij[r] += (double)acc[r][c] * dkl[c];
+                    jkl[c] += (double)acc[r][c] * dij[r];
                 }
-            }
 #pragma unroll
             for (int c = 0; c < NFCD; ++c) {
                 const double s = wave_sum(jkl[c]);
-                if ((t & 63) == 0) unsafeAtomicAdd(VJ + (size_t)(k0 + c / NFD) * nao + l0 + c % NFD, s);
+                if (lane == 0) unsafeAtomicAdd(VJ + (k0 + c / NFD) * nao + l0 + c % NFD, s);
             }
         }
         if (dok) {
             kdirty |= active;
-            const size_t ri = (size_t)i0 * nao, rj = (size_t)j0 * nao;
             double kil[NFA][NFD], kjl[NFB][NFD];
 #pragma unroll
             for (int x = 0; x < NFA; ++x)
@@ -281,34 +301,29 @@ jk_reg_kernel(const JKArgs a)
 #pragma unroll
             for (int ia = 0; ia < NFA; ++ia)
 #pragma unroll
-                for (int jb = 0; jb < NFB; ++jb) {
-                    const size_t rowi = ri + (size_t)ia * nao, rowj = rj + (size_t)jb * nao;
+                for (int jb = 0; jb < NFB; ++jb)
 #pragma unroll
-                    for (int kc = 0; kc < NFC; ++kc) {
-                        const double djk = D[rowj + k0 + kc], dik = D[rowi + k0 + kc];
+                    for (int kc = 0; kc < NFC; ++kc)
 #pragma unroll
                         for (int ld = 0; ld < NFD; ++ld) {
                             const double v = (double)acc[ia * NFB + jb][kc * NFD + ld];
-                            const double djl = D[rowj + l0 + ld], dil = D[rowi + l0 + ld];
-                            kik[ia][kc] += v * djl;
-                            kjk[jb][kc] += v * dil;
-                            kil[ia][ld] += v * djk;
-                            kjl[jb][ld] += v * dik;
+                            kik[ia][kc] += v * djl[jb][ld];
+                            kjk[jb][kc] += v * dil[ia][ld];
+                            kil[ia][ld] += v * djk[jb][kc];
+                            kjl[jb][ld] += v * dik[ia][kc];
                         }
-                    }
-                }
 #pragma unroll
             for (int x = 0; x < NFA; ++x)
 #pragma unroll
                 for (int d = 0; d < NFD; ++d) {
                     const double v = seg_scan(kil[x][d], lane, seg_head);
-                    if (seg_tail && pvalid && v != 0.0) unsafeAtomicAdd(VK + ri + (size_t)x * nao + l0 + d, v);
+                    if (seg_tail && pvalid && v != 0.0) unsafeAtomicAdd(VK + ri + x * nao + l0 + d, v);
                 }
             if (active) {
 #pragma unroll
                 for (int x = 0; x < NFB; ++x)
 #pragma unroll
-                    for (int d = 0; d < NFD; ++d) unsafeAtomicAdd(VK + rj + (size_t)x * nao + l0 + d, kjl[x][d]);
+                    for (int d = 0; d < NFD; ++d) unsafeAtomicAdd(VK + rj + x * nao + l0 + d, kjl[x][d]);
             }
         }
     }
@@ -317,20 +332,21 @@ jk_reg_kernel(const JKArgs a)
 #pragma unroll
         for (int x = 0; x < NFA; ++x)
 #pragma unroll
-            for (int c = 0; c < NFC; ++c) unsafeAtomicAdd(VK + (size_t)(i0 + x) * nao + cur_k0 + c, kik[x][c]);
+            for (int c = 0; c < NFC; ++c) unsafeAtomicAdd(VK + ri + x * nao + cur_k0 + c, kik[x][c]);
 #pragma unroll
         for (int x = 0; x < NFB; ++x)
 #pragma unroll
-            for (int c = 0; c < NFC; ++c) unsafeAtomicAdd(VK + (size_t)(j0 + x) * nao + cur_k0 + c, kjk[x][c]);
+            for (int c = 0; c < NFC; ++c) unsafeAtomicAdd(VK + rj + x * nao + cur_k0 + c, kjk[x][c]);
     }
     if (doj && pvalid) {
 #pragma unroll
         for (int r = 0; r < NFAB; ++r)
-            if (jij[r] != 0.0) unsafeAtomicAdd(VJ + (size_t)(i0 + r / NFB) * nao + j0 + r % NFB, jij[r]);
+            if (jij[r] != 0.0) unsafeAtomicAdd(VJ + ri + (r / NFB) * nao + j0 + r % NFB, jij[r]);
     }
     if (a.counter) {
         double cnt = wave_sum((double)nquart);
-        if ((t & 63) == 0 && cnt > 0) atomicAdd(a.counter, (unsigned long long)cnt);
+        if (lane == 0 && cnt > 0) atomicAdd(a.counter, (unsigned long long)cnt);
+        if (lane == 0 && a.slots) atomicAdd(a.slots, 64ull * nslots);
     }
 }
 

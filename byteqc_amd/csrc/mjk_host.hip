@@ -85,9 +85,14 @@ struct mjk_basis_prod {
     int *pair_i = nullptr, *pair_j = nullptr;
     double *pair_q = nullptr;
     double *a12 = nullptr, *e12 = nullptr, *x12 = nullptr, *y12 = nullptr, *z12 = nullptr;
+    int *pair_aoi = nullptr, *pair_aoj = nullptr;
+    double *pair_geo = nullptr;  // [6][n_pairs]: x_i, y_i, z_i, (R_i - R_j)_x,y,z
+    int4 *pair_int = nullptr;    // [n_pairs] {ish, jsh, ao_i, ao_j}
+    double *pair_dbl = nullptr;  // [n_pairs][8] {q, x_i, y_i, z_i, AB_x, AB_y, AB_z, 0}
+    double *prim_rec = nullptr;  // [n_prim_pairs][6] {a, e, Px, Py, Pz, 1/a}
     double *dm_cond = nullptr;            // nbas * nbas workspace
     unsigned long long *dm_max = nullptr;  // scalar (double bits)
-    unsigned long long *counters = nullptr;  // [0] fp64 quartets, [1] fp32 quartets
+    unsigned long long *counters = nullptr;  // [0] fp64 quartets, [1] fp32 quartets, [2] quartet slots
     long last_launches = 0;
 };
 
@@ -183,8 +188,10 @@ int mjk_init_basis_prod(mjk_basis_prod **pbp, double diag_fac, const int32_t *ba
     bp->nao = ao_loc[nbas];
     bp->ncptype = ncptype;
     bp->n_pairs = n_pairs;
-    std::vector<double> a12, e12, x12, y12, z12;
-    std::vector<int> pi(n_pairs), pj(n_pairs);
+    std::vector<double> a12, e12, x12, y12, z12, geo(6 * (size_t)n_pairs);
+    std::vector<int> pi(n_pairs), pj(n_pairs), paoi(n_pairs), paoj(n_pairs);
+    std::vector<int4> pint(n_pairs);
+    std::vector<double> pdbl(8 * (size_t)n_pairs, 0.0), prec;
     int prim_total = 0;
     for (int c = 0; c < ncptype; ++c) {
         ClassType ct{};
@@ -217,6 +224,19 @@ int mjk_init_basis_prod(mjk_basis_prod **pbp, double diag_fac, const int32_t *ba
             }
             pi[pr] = i;
             pj[pr] = j;
+            paoi[pr] = ao_loc[i];
+            paoj[pr] = ao_loc[j];
+            geo[0 * (size_t)n_pairs + pr] = x[i];
+            geo[1 * (size_t)n_pairs + pr] = y[i];
+            geo[2 * (size_t)n_pairs + pr] = z[i];
+            geo[3 * (size_t)n_pairs + pr] = x[i] - x[j];
+            geo[4 * (size_t)n_pairs + pr] = y[i] - y[j];
+            geo[5 * (size_t)n_pairs + pr] = z[i] - z[j];
+            pint[pr] = make_int4(i, j, ao_loc[i], ao_loc[j]);
+            double *pd = &pdbl[8 * (size_t)pr];
+            pd[0] = pair_q[pr];
+            pd[1] = x[i]; pd[2] = y[i]; pd[3] = z[i];
+            pd[4] = x[i] - x[j]; pd[5] = y[i] - y[j]; pd[6] = z[i] - z[j];
             const double *ai = env + bi[PTR_EXP], *aj = env + bj[PTR_EXP];
             const double *ci = env + bi[PTR_COEFF], *cj = env + bj[PTR_COEFF];
             const double rx = x[i] - x[j], ry = y[i] - y[j], rz = z[i] - z[j];
@@ -230,6 +250,7 @@ int mjk_init_basis_prod(mjk_basis_prod **pbp, double diag_fac, const int32_t *ba
                     x12.push_back((ai[ip] * x[i] + aj[jp] * x[j]) / aij);
                     y12.push_back((ai[ip] * y[i] + aj[jp] * y[j]) / aij);
                     z12.push_back((ai[ip] * z[i] + aj[jp] * z[j]) / aij);
+                    prec.insert(prec.end(), {aij, e12.back(), x12.back(), y12.back(), z12.back(), 1.0 / aij});
                 }
         }
         prim_total += ct.npairs * ct.nprim12;
@@ -240,12 +261,14 @@ int mjk_init_basis_prod(mjk_basis_prod **pbp, double diag_fac, const int32_t *ba
 #define UP(ptr, vec) if ((e = bp->pool.upload(&bp->ptr, vec.data(), vec.size())) != hipSuccess) goto oom;
     UP(bx, x) UP(by, y) UP(bz, z) UP(ao_loc, ao_loc) UP(pair_i, pi) UP(pair_j, pj)
     UP(a12, a12) UP(e12, e12) UP(x12, x12) UP(y12, y12) UP(z12, z12)
+    UP(pair_aoi, paoi) UP(pair_aoj, paoj) UP(pair_geo, geo)
+    UP(pair_int, pint) UP(pair_dbl, pdbl) UP(prim_rec, prec)
 #undef UP
     if ((e = bp->pool.upload(&bp->pair_q, pair_q, (size_t)n_pairs)) != hipSuccess) goto oom;
     if ((e = bp->pool.alloc(&bp->dm_cond, (size_t)nbas * nbas)) != hipSuccess) goto oom;
     if ((e = bp->pool.alloc(&bp->dm_max, 1)) != hipSuccess) goto oom;
-    if ((e = bp->pool.alloc(&bp->counters, 2)) != hipSuccess) goto oom;
-    if ((e = hipMemset(bp->counters, 0, 2 * sizeof(unsigned long long))) != hipSuccess) goto oom;
+    if ((e = bp->pool.alloc(&bp->counters, 4)) != hipSuccess) goto oom;
+    if ((e = hipMemset(bp->counters, 0, 4 * sizeof(unsigned long long))) != hipSuccess) goto oom;
     *pbp = bp;
     return MJK_OK;
 oom:
@@ -279,6 +302,18 @@ PairSet pair_set(const mjk_basis_prod *bp, const ClassType &ct)
     s.px = bp->x12 + ct.prim_off;
     s.py = bp->y12 + ct.prim_off;
     s.pz = bp->z12 + ct.prim_off;
+    s.aoi = bp->pair_aoi + ct.pair_off;
+    s.aoj = bp->pair_aoj + ct.pair_off;
+    const size_t np = bp->n_pairs;
+    s.cx = bp->pair_geo + 0 * np + ct.pair_off;
+    s.cy = bp->pair_geo + 1 * np + ct.pair_off;
+    s.cz = bp->pair_geo + 2 * np + ct.pair_off;
+    s.abx = bp->pair_geo + 3 * np + ct.pair_off;
+    s.aby = bp->pair_geo + 4 * np + ct.pair_off;
+    s.abz = bp->pair_geo + 5 * np + ct.pair_off;
+    s.pint = bp->pair_int + ct.pair_off;
+    s.pdbl = bp->pair_dbl + 8 * (size_t)ct.pair_off;
+    s.prim = bp->prim_rec + 6 * (size_t)ct.prim_off;
     s.npairs = ct.npairs;
     s.nprim = ct.nprim12;
     return s;
@@ -312,6 +347,7 @@ int run_class_pair(mjk_basis_prod *bp, double *vj, double *vk, const double *dm,
     a.lo = pc > 0 ? pc : 0.0;
     a.hi = INFINITY;
     a.counter = bp->counters;
+    a.slots = bp->counters + 2;
     hipError_t e = launch_class(bra->l_bra, bra->l_ket, ket->l_bra, ket->l_ket, false, a, st, &nt);
     if (e != hipSuccess)
         return fail(MJK_ERR_HIP, std::string("mjk_build_jk: launch of (") + "spdf"[bra->l_bra] + "spdf"[bra->l_ket] + "|" +
@@ -371,7 +407,7 @@ int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *d
     hipError_t e;
     if (vj && (e = launch_fill(vj, n, 0.0, st)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: zero vj");
     if (vk && (e = launch_fill(vk, n, 0.0, st)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: zero vk");
-    if ((e = launch_fill_u64(bp->counters, 2, st)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: counters");
+    if ((e = launch_fill_u64(bp->counters, 4, st)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: counters");
     bp->last_launches = 0;
     if (!vj && !vk) return MJK_OK;
     const bool dens = screen->density_screen != 0;
@@ -399,12 +435,13 @@ int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *d
 int mjk_last_stats(mjk_basis_prod *bp, double *out)
 {
     if (!bp || !out) return fail(MJK_ERR_ARG, "mjk_last_stats: bad argument");
-    unsigned long long c[2];
+    unsigned long long c[3];
     hipError_t e = hipMemcpy(c, bp->counters, sizeof(c), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "mjk_last_stats");
     out[0] = (double)c[0];
     out[1] = (double)c[1];
     out[2] = (double)bp->last_launches;
+    out[3] = (double)c[2];
     return MJK_OK;
 }
 

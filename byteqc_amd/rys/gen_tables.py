@@ -162,7 +162,16 @@ def _fmt(v):
     return repr(float(v))
 
 
-def emit(path, t64, t32):
+BOYS_H, BOYS_TMAX, BOYS_K = 0.1, 40.0, 10  # grid step, range, highest order stored
+
+
+def boys_grid():
+    """F_0..F_BOYS_K at T = 0, h, 2h, ... < BOYS_TMAX (mpmath), shape (npts, BOYS_K+1)."""
+    n = int(round(BOYS_TMAX / BOYS_H))
+    return np.array([[float(v) for v in boys_mp(i * BOYS_H, BOYS_K)] for i in range(n + 1)])
+
+
+def emit(path, t64, t32, grid):
     lines = [
         '// Generated by byteqc_amd/rys/gen_tables.py -- do not edit by hand.',
         '// Piecewise Chebyshev fits of Rys nodes x = t^2 and weights w for exp(-T t^2)',
@@ -201,6 +210,15 @@ def emit(path, t64, t32):
     lines.append('// half-range Gauss-Hermite: s_i^2 and W_i, N = 1..NMAX concatenated')
     lines.append('RYS_STORAGE double rys_herm_s2[%d] = {%s};' % (len(hs), ', '.join(map(_fmt, hs))))
     lines.append('RYS_STORAGE double rys_herm_w[%d] = {%s};' % (len(hw), ', '.join(map(_fmt, hw))))
+    lines.append('// Boys F_k(T_i), T_i = i * RYS_BOYS_H, k = 0..RYS_BOYS_K: the one-root rule uses')
+    lines.append('// F_n(T_i - d) = sum_k F_(n+k)(T_i) d^k / k! with |d| <= h/2')
+    lines.append('#define RYS_BOYS_H %r' % BOYS_H)
+    lines.append('#define RYS_BOYS_TMAX %r' % BOYS_TMAX)
+    lines.append('#define RYS_BOYS_K %d' % BOYS_K)
+    lines.append('RYS_STORAGE double rys_boys_grid[%d] = {' % grid.size)
+    for row in grid:
+        lines.append('  ' + ', '.join(map(_fmt, row)) + ',')
+    lines.append('};')
     lines.append('RYS_STORAGE double rys_tab64[%d] = {' % off64[-1])
     for n in range(1, NMAX + 1):
         lines.append('  // N = %d' % n)
@@ -231,8 +249,10 @@ def main(argv=None):
             print(f'N={n}: fp64 fit rel err x={e64[0]:.2e} w={e64[1]:.2e}; '
                   f'fp32 fit x={e32[0]:.2e} w={e32[1]:.2e}; asymptote at T={tmax(n)}: '
                   f'x={ea[0]:.2e} w={ea[1]:.2e}', flush=True)
-    emit(out, t64, t32)
+    grid = boys_grid()
+    emit(out, t64, t32, grid)
     np.savez_compressed(os.path.join(os.path.dirname(out), 'rys_tables.npz'),
+                        boys_grid=grid,
                         **{f'tab64_{n}': t64[n] for n in t64}, **{f'tab32_{n}': t32[n] for n in t32})
     print('wrote', out)
 

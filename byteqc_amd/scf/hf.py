@@ -212,9 +212,10 @@ class _VHFOpt:
         return int(self.bas_pairs_locs[-1])
 
     def stats(self):
-        out = np.zeros(3)
+        out = np.zeros(4)
         mjk.check(mjk.load().mjk_last_stats(self.bp, mjk.ptr(out)), 'mjk_last_stats')
-        return {'quartets_fp64': int(out[0]), 'quartets_fp32': int(out[1]), 'launches': int(out[2])}
+        return {'quartets_fp64': int(out[0]), 'quartets_fp32': int(out[1]), 'launches': int(out[2]),
+                'slots': int(out[3])}
 
 
 def _morton_keys(atm, env, bas, cell=2.0):

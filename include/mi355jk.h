@@ -97,7 +97,9 @@ int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *d
                      int nao, int n_dm, const mjk_screen *screen, void *stream);
 
 /* Counters of the last mjk_build_jk_all on this cache (host values, synchronises):
- * out[0] quartets computed in FP64, out[1] in FP32, out[2] kernel launches. */
+ * out[0] quartets computed in FP64, out[1] in FP32, out[2] kernel launches, out[3] quartet
+ * slots the register and group kernels executed (lanes or lane groups times ket
+ * iterations), so out[0..1] / out[3] is their lane utilisation. */
 int mjk_last_stats(mjk_basis_prod *bp, double *out);
 
 /* Overlap, kinetic and nuclear-attraction matrices (host arrays, nao x nao each) of a

@@ -77,3 +77,27 @@ def test_header_matches_npz():
     assert f'#define RYS_DEG64 {g.DEG64}' in text and f'#define RYS_NMAX {g.NMAX}' in text
     n64 = sum(TAB[f'tab64_{n}'].size for n in range(1, g.NMAX + 1))
     assert f'rys_tab64[{n64}]' in text
+
+
+def root1_grid(T):
+    """CPU evaluation of the kernels' one-root rule: Taylor series on the Boys grid below
+    RYS_BOYS_TMAX, the asymptote above it."""
+    if T >= g.BOYS_TMAX:
+        return 0.5 / T, 0.5 * np.sqrt(np.pi / T)
+    grid = TAB['boys_grid'].reshape(-1, g.BOYS_K + 1)
+    i = int(T / g.BOYS_H + 0.5)
+    d = i * g.BOYS_H - T
+    fact = np.cumprod([1.0] + list(range(1, g.BOYS_K)))
+    f0 = sum(grid[i, k] * d ** k / fact[k] for k in range(g.BOYS_K))
+    f1 = sum(grid[i, k + 1] * d ** k / fact[k] for k in range(g.BOYS_K))
+    return f1 / f0, f0
+
+
+def test_boys_grid_root1():
+    rng = np.random.default_rng(7)
+    for T in np.concatenate([[0.0, 1e-9, 0.05, 0.0500001, g.BOYS_TMAX - 1e-9, g.BOYS_TMAX, 77.0],
+                             rng.uniform(0, g.BOYS_TMAX + 5, 12)]):
+        x, w = root1_grid(T)
+        f0, f1 = boys_quad(0, T), boys_quad(1, T)
+        assert abs(w - f0) <= 4e-16 * f0 + 1e-300, T
+        assert abs(x - f1 / f0) <= 1e-15 * (f1 / f0), T
