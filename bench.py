@@ -17,6 +17,10 @@ Also reported:
                 screening and density, timed on a systematic sample of bra-pair rows and
                 scaled to the full build
   scf           (--scf) RHF time-to-1e-8 Eh from the SAD guess
+  batch         (--batch N) N seeded water clusters of 1-32 molecules, 6-31G*, random PSD
+                densities (D = 2 Q Q^T, trace = electron count): all N J/K builds as a
+                per-molecule loop of mjk_build_jk_all, as one batched build, as a replayed
+                hipGraph, and as a replayed graph after the ket-chunk autotuner
 """
 from __future__ import annotations
 
@@ -37,6 +41,74 @@ FP64_PEAK_TFLOPS = 78.6
 
 def log(msg):
     print(f'[bench] {msg}', file=sys.stderr, flush=True)
+
+
+def run_batch(n_mol, reps, torch):
+    """Seconds for all n_mol J/K builds: per-molecule loop vs batched launches."""
+    from byteqc_amd.gto.clusters import water_cluster
+    from byteqc_amd.guess.train import box_for
+    from byteqc_amd.scf import hf
+    from byteqc_amd.scf.batch import BatchJK
+    rng = np.random.default_rng(2341)
+    sizes = rng.integers(1, 33, n_mol)
+    mols = [water_cluster(int(n), box_for(int(n)), 3000 + i, basis='6-31gs') for i, n in enumerate(sizes)]
+    dms = []
+    for m in mols:
+        q, _ = np.linalg.qr(rng.normal(size=(m.nao, m.nelectron // 2)))
+        dms.append(2.0 * q @ q.T)
+    t0 = time.perf_counter()
+    b = BatchJK(mols, use_graph=True)
+    setup = time.perf_counter() - t0
+    flat = b.to_cart(dms)
+    log(f'batch: {n_mol} clusters, {int(sizes.sum())} waters, nao_cart total {int(b.nao_cart.sum())}, '
+        f'{b.ncp} class pairs, setup {setup:.1f} s')
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n
+
+    out = {'n_mol': n_mol, 'waters': int(sizes.sum()), 'basis': '6-31G*', 'setup_s': setup}
+    vj = torch.empty_like(flat)
+    vk = torch.empty_like(flat)
+    out['batched_s'] = timed(lambda: b.build_cart(flat, vj, vk, use_graph=False), reps)
+    out['graph_s'] = timed(lambda: b.build_cart(flat, vj, vk, use_graph=True), reps)
+    t0 = time.perf_counter()
+    table = b.tune(flat, reps=2)
+    out['tune_s'] = time.perf_counter() - t0
+    out['graph_tuned_s'] = timed(lambda: b.build_cart(flat, vj, vk, use_graph=True), reps)
+    out['tuned_kc'] = {k: v[0] for k, v in table.items()}
+    st = b.stats()
+    out['quartets'] = st['quartets_fp64'] + st['quartets_fp32']
+    ref_j, ref_k = vj.clone(), vk.clone()
+    b.close()
+    opts = [hf._VHFOpt(m).build(1e-13) for m in mols]
+    dmc = [flat[b.dm_off[i]:b.dm_off[i] + int(b.nao_cart[i]) ** 2].reshape(1, int(b.nao_cart[i]), -1)
+           for i in range(n_mol)]
+    res = []
+
+    def loop():
+        res.clear()
+        for o, d in zip(opts, dmc):
+            res.append(hf.jk_device(o, d, True, True, 0.0, density_screen=False))
+
+    out['loop_s'] = timed(loop, reps)
+    err = 0.0
+    for i, (j, k) in enumerate(res):
+        n = int(b.nao_cart[i])
+        sl = slice(int(b.dm_off[i]), int(b.dm_off[i]) + n * n)
+        err = max(err, float((j.reshape(-1) - ref_j[sl]).abs().max()), float((k.reshape(-1) - ref_k[sl]).abs().max()))
+    out['max_abs_diff_vs_loop'] = err
+    out['speedup_graph_tuned_vs_loop'] = out['loop_s'] / out['graph_tuned_s']
+    for o in opts:
+        o.clear()
+    log(f'batch: loop {out["loop_s"]:.3f} s, batched {out["batched_s"]:.3f} s, graph {out["graph_s"]:.3f} s, '
+        f'graph+tuned {out["graph_tuned_s"]:.3f} s (tuning {out["tune_s"]:.1f} s), max |diff| {err:.1e}')
+    return out
 
 
 def per_class_profile(vhfopt, dmc, vj, vk, screen, torch, lib, mjk):
@@ -136,6 +208,7 @@ def main(argv=None):
     ap.add_argument('--no-classes', action='store_true')
     ap.add_argument('--scf', action='store_true')
     ap.add_argument('--classes-out', default=None, help='write every class timing to this JSON file')
+    ap.add_argument('--batch', type=int, default=0, help='also time N batched 6-31G* cluster builds')
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f'bench.py measures one GPU only (got --gpus {args.gpus}); the J/K build has no '
@@ -254,6 +327,10 @@ def main(argv=None):
         r['mixed_delta_e'] = r['mixed_sad']['e_tot'] - r['fp64_sad']['e_tot']
         r['speedup'] = r['fp64_sad']['time_s'] / r['mixed_sad']['time_s']
         log(f'scf mixed {r["mixed_sad"]}  dE={r["mixed_delta_e"]:.2e} speedup={r["speedup"]:.2f}')
+    if args.batch:
+        vhfopt.clear()
+        torch.cuda.empty_cache()
+        result['batch'] = run_batch(args.batch, 3, torch)
     print(json.dumps(result), flush=True)
     return 0
 

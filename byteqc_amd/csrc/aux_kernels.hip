@@ -172,6 +172,59 @@ hipError_t launch_symmetrize(double *v, int n_dm, int nao, double f, hipStream_t
     return hipGetLastError();
 }
 
+// J = f (V + V^T) per molecule of a batch; blockIdx.y is the molecule
+__global__ void symmetrize_batch_kernel(double *v, const MolRec *mols, double f)
+{
+    const MolRec m = mols[blockIdx.y];
+    const size_t nn = (size_t)m.nao * m.nao;
+    double *V = v + m.off;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < nn; t += (size_t)gridDim.x * blockDim.x) {
+        const int a = (int)(t / m.nao), b = (int)(t % m.nao);
+        if (b > a) continue;
+        const double s = f * (V[(size_t)a * m.nao + b] + V[(size_t)b * m.nao + a]);
+        V[(size_t)a * m.nao + b] = s;
+        V[(size_t)b * m.nao + a] = s;
+    }
+}
+
+hipError_t launch_symmetrize_batch(double *v, const MolRec *mols, int n_mol, int max_nao, double f, hipStream_t st)
+{
+    if (n_mol <= 0) return hipSuccess;
+    const size_t nn = (size_t)max_nao * max_nao;
+    unsigned gx = (unsigned)((nn + 255) / 256);
+    if (gx > 1024) gx = 1024;
+    if (gx == 0) gx = 1;
+    hipLaunchKernelGGL(symmetrize_batch_kernel, dim3(gx, (unsigned)n_mol), dim3(256), 0, st, v, mols, f);
+    return hipGetLastError();
+}
+
+int class_kernel_info(int la, int lb, int lc, int ld, int *bra_per_tile)
+{
+#define MJK_INFO(A, B, C, D)                                                                     \
+    if constexpr (canon(A, B, C, D)) {                                                           \
+        if (la == A && lb == B && lc == C && ld == D) {                                          \
+            if constexpr (Cfg<A, B, C, D>::REG) {                                                \
+                *bra_per_tile = REG_BRA;                                                         \
+                return 0;                                                                        \
+            } else if constexpr (GCfg<A, B, C, D>::OK) {                                         \
+                *bra_per_tile = GCfg<A, B, C, D>::W;                                             \
+                return 1;                                                                        \
+            } else {                                                                             \
+                *bra_per_tile = Cfg<A, B, C, D>::BP;                                             \
+                return 2;                                                                        \
+            }                                                                                    \
+        }                                                                                        \
+    }
+#define MJK_INFO_D(A, B, C) MJK_INFO(A, B, C, 0) MJK_INFO(A, B, C, 1) MJK_INFO(A, B, C, 2) MJK_INFO(A, B, C, 3)
+#define MJK_INFO_C(A, B) MJK_INFO_D(A, B, 0) MJK_INFO_D(A, B, 1) MJK_INFO_D(A, B, 2) MJK_INFO_D(A, B, 3)
+    MJK_INFO_C(0, 0) MJK_INFO_C(1, 0) MJK_INFO_C(1, 1) MJK_INFO_C(2, 0) MJK_INFO_C(2, 1)
+    MJK_INFO_C(2, 2) MJK_INFO_C(3, 0) MJK_INFO_C(3, 1) MJK_INFO_C(3, 2) MJK_INFO_C(3, 3)
+#undef MJK_INFO_C
+#undef MJK_INFO_D
+#undef MJK_INFO
+    return -1;
+}
+
 hipError_t launch_class(int la, int lb, int lc, int ld, bool fp32, const JKArgs &a, hipStream_t st,
                         long *nt)
 {

@@ -67,8 +67,21 @@ struct PairSet {
     int npairs, nprim;
 };
 
+// One molecule of a batched build: its D, J and K start at element `off` of the batch
+// arrays (nao x nao each) and its shells' AO indices start at ao_base.
+struct MolRec {
+    long off;
+    int nao, ao_base;
+};
+
 struct JKArgs {
     PairSet bra, ket;
+    // batched builds: tile t covers bra pairs [x, y) and ket pairs [z, w) of tiles[t], all
+    // of molecule tile_mol[t]; null for a single molecule (tiles from the class sizes)
+    const int4 *tiles;
+    const int *tile_mol;
+    const MolRec *mols;
+    long ndesc;             // number of descriptors in tiles
     const double *bx, *by, *bz;
     const int *ao_loc;
     const double *dm;
@@ -77,7 +90,9 @@ struct JKArgs {
     unsigned long long *counter;  // quartets computed (optional)
     unsigned long long *slots;    // lane slots of executed ket iterations (optional)
     double tol;             // keep quartets with bound > tol
-    double lo, hi;          // this launch takes lo <= bound < hi
+    double lo, hi;          // precision split: the register and group kernels take the tiles whose
+                            // largest bound m has lo <= m < hi; the block-LDS kernel takes the
+                            // quartets with lo <= bound < hi
     int nbas, nao, n_dm;
     const double *dm_max;   // device scalar: global max |D| (null: 1)
     long tile_base;         // first tile of this launch

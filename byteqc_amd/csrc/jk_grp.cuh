@@ -70,18 +70,36 @@ jk_grp_kernel(const JKArgs a)
     __shared__ double kbuf[W][C::KBUF];
 
     const long tile = a.tile_base + (long)blockIdx.x;
-    const int bi = (int)(tile / a.nb_ket), ki = (int)(tile % a.nb_ket);
     const int lane = threadIdx.x;
     const int g = lane / S, sub = lane % S;
     const bool lane_ok = g < W;
     const int gg = lane_ok ? g : 0;
-    const int p = bi * W + g;
-    const int q_begin = ki * a.kc;
-    if (a.same && q_begin > bi * W + W - 1) return;  // the tile lies above the triangle
-    int q_end = q_begin + a.kc;
-    if (q_end > a.ket.npairs) q_end = a.ket.npairs;
-
-    const bool pvalid = lane_ok && p < a.bra.npairs;
+    int p, q_begin, q_end;
+    bool pvalid;
+    const double *D = a.dm;
+    double *VJ = a.vj, *VK = a.vk;
+    int nao = a.nao, ao_base = 0;
+    if (a.tiles) {
+        const int4 d = a.tiles[tile];
+        p = d.x + g;
+        pvalid = lane_ok && p < d.y;
+        q_begin = d.z;
+        q_end = d.w;
+        const MolRec m = a.mols[a.tile_mol[tile]];
+        D += m.off;
+        VJ = VJ ? VJ + m.off : nullptr;
+        VK = VK ? VK + m.off : nullptr;
+        nao = m.nao;
+        ao_base = m.ao_base;
+    } else {
+        const int bi = (int)(tile / a.nb_ket), ki = (int)(tile % a.nb_ket);
+        p = bi * W + g;
+        q_begin = ki * a.kc;
+        if (a.same && q_begin > bi * W + W - 1) return;  // the tile lies above the triangle
+        q_end = q_begin + a.kc;
+        if (q_end > a.ket.npairs) q_end = a.ket.npairs;
+        pvalid = lane_ok && p < a.bra.npairs;
+    }
     const int pp = pvalid ? p : 0;
     const int4 b4 = a.bra.pint[pp];
     const double *bd = a.bra.pdbl + 8 * pp;
@@ -97,14 +115,13 @@ jk_grp_kernel(const JKArgs a)
             qk = fmax(qk, __shfl_xor(qk, off, 64));
             qb = fmax(qb, __shfl_xor(qb, off, 64));
         }
-        if (!(qb * qk * dmax > a.tol && qb * qk * dmax >= a.lo)) return;  // wave-uniform
+        // per-tile precision split: lo <= (largest bound of the tile) < hi
+        const double m = qb * qk * dmax;
+        if (!(m > a.tol && m >= a.lo && m < a.hi)) return;  // wave-uniform
     }
     const double A[3] = {bd[1], bd[2], bd[3]};
     const double AB[3] = {bd[4], bd[5], bd[6]};
-    const int nao = a.nao;
-    const int i0 = b4.z, j0 = b4.w;
-    const double *D = a.dm;
-    double *VJ = a.vj, *VK = a.vk;
+    const int i0 = b4.z - ao_base, j0 = b4.w - ao_base;
     const bool doj = VJ != nullptr, dok = VK != nullptr;
 
     // this lane's bra rows
@@ -136,8 +153,8 @@ jk_grp_kernel(const JKArgs a)
         const int4 k4 = a.ket.pint[q];
         const int ksh = __builtin_amdgcn_readfirstlane(k4.x);
         const int lsh = __builtin_amdgcn_readfirstlane(k4.y);
-        const int k0 = __builtin_amdgcn_readfirstlane(k4.z);
-        const int l0 = __builtin_amdgcn_readfirstlane(k4.w);
+        const int k0 = __builtin_amdgcn_readfirstlane(k4.z) - ao_base;
+        const int l0 = __builtin_amdgcn_readfirstlane(k4.w) - ao_base;
         const double *kd = a.ket.pdbl + 8 * q;
         if (ksh != cur_k) {
             if (dok && cur_k >= 0 && pvalid) {
@@ -165,7 +182,7 @@ jk_grp_kernel(const JKArgs a)
                                            fmax(dc[jsh * nb + ksh], dc[jsh * nb + lsh])));
                 bound *= d;
             }
-            active = bound > a.tol && bound >= a.lo && bound < a.hi;
+            active = bound > a.tol;
         }
         if (!__builtin_amdgcn_ballot_w64(active)) continue;  // wave-uniform
         if (active && sub == 0) ++nquart;

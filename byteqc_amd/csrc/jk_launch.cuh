@@ -27,7 +27,11 @@ hipError_t launch(JKArgs a, hipStream_t st, long *ntiles_out)
         while (kc > 1 && nb_bra * ((a.ket.npairs + kc - 1) / kc) < 2048) kc >>= 1;
         return kc;
     };
-    if constexpr (C::REG) {
+    if (a.tiles) {
+        // batched build: the host enumerated the tiles (register and group kernels only)
+        if constexpr (!(C::REG || GCfg<LA, LB, LC, LD>::OK)) return hipErrorNotSupported;
+        ntiles = a.ndesc;
+    } else if constexpr (C::REG) {
         a.nb_bra = (a.bra.npairs + REG_BRA - 1) / REG_BRA;
         a.kc = pick_kc(a.nb_bra, REG_KC);
         a.nb_ket = (a.ket.npairs + a.kc - 1) / a.kc;

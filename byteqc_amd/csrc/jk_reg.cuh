@@ -64,27 +64,46 @@ jk_reg_kernel(const JKArgs a)
 
     // ---- tile
     const long tile = a.tile_base + (long)blockIdx.x;
-    int bi, ki;
-    if (!a.same) {
-        bi = (int)(tile / a.nb_ket);
-        ki = (int)(tile % a.nb_ket);
-    } else {
-        long lo = 0, hi = a.nb_bra;
-        while (hi - lo > 1) {
-            const long mid = (lo + hi) / 2;
-            if (reg_tri_tiles(mid, a.nb_ket, a.kc) <= tile) lo = mid; else hi = mid;
-        }
-        bi = (int)lo;
-        ki = (int)(tile - reg_tri_tiles(lo, a.nb_ket, a.kc));
-    }
     const int t = threadIdx.x;
     const int lane = t & 63;
-    const int p = bi * REG_BRA + t;
-    const int q_begin = ki * a.kc;
-    int q_end = q_begin + a.kc;
-    if (q_end > a.ket.npairs) q_end = a.ket.npairs;
+    int p, q_begin, q_end;
+    bool pvalid;
+    const double *__restrict__ D = a.dm;
+    double *VJ = a.vj, *VK = a.vk;
+    int nao = a.nao, ao_base = 0;
+    if (a.tiles) {
+        const int4 d = a.tiles[tile];
+        p = d.x + t;
+        pvalid = p < d.y;
+        q_begin = d.z;
+        q_end = d.w;
+        const MolRec m = a.mols[a.tile_mol[tile]];
+        D += m.off;
+        VJ = VJ ? VJ + m.off : nullptr;
+        VK = VK ? VK + m.off : nullptr;
+        nao = m.nao;
+        ao_base = m.ao_base;
+    } else {
+        int bi, ki;
+        if (!a.same) {
+            bi = (int)(tile / a.nb_ket);
+            ki = (int)(tile % a.nb_ket);
+        } else {
+            long lo = 0, hi = a.nb_bra;
+            while (hi - lo > 1) {
+                const long mid = (lo + hi) / 2;
+                if (reg_tri_tiles(mid, a.nb_ket, a.kc) <= tile) lo = mid; else hi = mid;
+            }
+            bi = (int)lo;
+            ki = (int)(tile - reg_tri_tiles(lo, a.nb_ket, a.kc));
+        }
+        p = bi * REG_BRA + t;
+        pvalid = p < a.bra.npairs;
+        q_begin = ki * a.kc;
+        q_end = q_begin + a.kc;
+        if (q_end > a.ket.npairs) q_end = a.ket.npairs;
+    }
 
-    const bool pvalid = p < a.bra.npairs;
     const int pp = pvalid ? p : 0;
     const int4 b4 = a.bra.pint[pp];
     const double *bd = a.bra.pdbl + 8 * pp;
@@ -101,8 +120,12 @@ jk_reg_kernel(const JKArgs a)
             qk = fmax(qk, __shfl_xor(qk, off, 64));
             qb = fmax(qb, __shfl_xor(qb, off, 64));
         }
-        const int live = qb * qk * dmax > a.tol && qb * qk * dmax >= a.lo;
-        if (!__syncthreads_or(live)) return;
+        // the precision split is per tile: this launch takes the tiles whose largest bound m
+        // satisfies lo <= m < hi, and computes every quartet of such a tile
+        const double m = qb * qk * dmax;
+        if (!__syncthreads_or(m > a.tol)) return;
+        if (!__syncthreads_or(m >= a.lo)) return;
+        if (__syncthreads_or(m >= a.hi)) return;
     }
     // lanes holding bra pairs with the same shell i form runs: K_il is summed over a run
     int seg_head, seg_tail;
@@ -118,10 +141,7 @@ jk_reg_kernel(const JKArgs a)
     }
     const double A[3] = {bd[1], bd[2], bd[3]};
     const double AB[3] = {bd[4], bd[5], bd[6]};
-    const int nao = a.nao;
-    const int i0 = b4.z, j0 = b4.w;
-    const double *__restrict__ D = a.dm;
-    double *VJ = a.vj, *VK = a.vk;
+    const int i0 = b4.z - ao_base, j0 = b4.w - ao_base;
     const bool doj = VJ != nullptr, dok = VK != nullptr;
     const int ri = i0 * nao, rj = j0 * nao;  // row offsets (nao^2 < 2^31)
 
@@ -150,8 +170,8 @@ jk_reg_kernel(const JKArgs a)
         const int4 k4 = a.ket.pint[q];
         const int ksh = __builtin_amdgcn_readfirstlane(k4.x);
         const int lsh = __builtin_amdgcn_readfirstlane(k4.y);
-        const int k0 = __builtin_amdgcn_readfirstlane(k4.z);
-        const int l0 = __builtin_amdgcn_readfirstlane(k4.w);
+        const int k0 = __builtin_amdgcn_readfirstlane(k4.z) - ao_base;
+        const int l0 = __builtin_amdgcn_readfirstlane(k4.w) - ao_base;
         const double *kd = a.ket.pdbl + 8 * q;
         const double qq = kd[0];
         const double Cc[3] = {kd[1], kd[2], kd[3]};
@@ -206,7 +226,7 @@ jk_reg_kernel(const JKArgs a)
                                            fmax(dc[jsh * nb + ksh], dc[jsh * nb + lsh])));
                 bound *= d;
             }
-            active = bound > a.tol && bound >= a.lo && bound < a.hi;
+            active = bound > a.tol;
         }
         if (!__builtin_amdgcn_ballot_w64(active)) continue;  // wave-uniform skip
         ++nslots;

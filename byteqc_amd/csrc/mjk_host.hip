@@ -30,6 +30,12 @@ int hip_fail(hipError_t e, const char *where)
     return fail(MJK_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
+}  // namespace
+
+int mjk::report_error(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIP_TRY(call)                                           \
     do {                                                        \
         hipError_t e_ = (call);                                 \
@@ -265,7 +271,6 @@ int mjk_init_basis_prod(mjk_basis_prod **pbp, double diag_fac, const int32_t *ba
     UP(pair_int, pint) UP(pair_dbl, pdbl) UP(prim_rec, prec)
 #undef UP
     if ((e = bp->pool.upload(&bp->pair_q, pair_q, (size_t)n_pairs)) != hipSuccess) goto oom;
-    if ((e = bp->pool.alloc(&bp->dm_cond, (size_t)nbas * nbas)) != hipSuccess) goto oom;
     if ((e = bp->pool.alloc(&bp->dm_max, 1)) != hipSuccess) goto oom;
     if ((e = bp->pool.alloc(&bp->counters, 4)) != hipSuccess) goto oom;
     if ((e = hipMemset(bp->counters, 0, 4 * sizeof(unsigned long long))) != hipSuccess) goto oom;
@@ -376,6 +381,18 @@ int check_common(mjk_basis_prod *bp, const double *dm, int nao, int n_dm, const 
     return MJK_OK;
 }
 
+// The nbas x nbas block maxima are allocated on the first density-screened build (a
+// concatenated batch basis never needs them).
+int ensure_dm_cond(mjk_basis_prod *bp)
+{
+    if (bp->dm_cond) return MJK_OK;
+    hipError_t e = bp->pool.alloc(&bp->dm_cond, (size_t)bp->nbas * bp->nbas);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? MJK_ERR_OOM : MJK_ERR_HIP,
+                    std::string("density screen workspace: ") + hipGetErrorString(e));
+    return MJK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -390,6 +407,9 @@ int mjk_build_jk(mjk_basis_prod *bp, double *vj, double *vk, const double *dm, i
     if (!vj && !vk) return MJK_OK;
     hipStream_t st = (hipStream_t)stream;
     const bool dens = screen->density_screen != 0;
+    if (screen->density_screen == 2 && !bp->dm_cond)
+        return fail(MJK_ERR_ARG, "mjk_build_jk: density_screen 2 needs an earlier density_screen 1 call");
+    if (screen->density_screen == 1 && (rc = ensure_dm_cond(bp)) != MJK_OK) return rc;
     if (screen->density_screen == 1) {
         hipError_t e = launch_dm_cond(dm, n_dm, nao, bp->ao_loc, bp->nbas, bp->dm_cond, bp->dm_max, st);
         if (e != hipSuccess) return hip_fail(e, "mjk_build_jk: density screen");
@@ -411,6 +431,7 @@ int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *d
     bp->last_launches = 0;
     if (!vj && !vk) return MJK_OK;
     const bool dens = screen->density_screen != 0;
+    if (dens && (rc = ensure_dm_cond(bp)) != MJK_OK) return rc;
     if (dens && (e = launch_dm_cond(dm, n_dm, nao, bp->ao_loc, bp->nbas, bp->dm_cond, bp->dm_max, st)) != hipSuccess)
         return hip_fail(e, "mjk_build_jk_all: density screen");
     // heavy classes first
@@ -430,6 +451,355 @@ int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *d
     if (vj && (e = launch_symmetrize(vj, n_dm, nao, 2.0, st)) != hipSuccess) return hip_fail(e, "symmetrize vj");
     if (vk && (e = launch_symmetrize(vk, n_dm, nao, 1.0, st)) != hipSuccess) return hip_fail(e, "symmetrize vk");
     return MJK_OK;
+}
+
+// ---- batched builds ---------------------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+struct BatchCP {
+    int cb = 0, ck = 0;      // bra and ket class (bra has at least as many Cartesian products)
+    int l[4] = {0, 0, 0, 0};
+    int kind = 0;            // 0 register, 1 group, 2 block-LDS (single molecule only)
+    int bra_tile = 1, kc = 0;
+    bool same = false;
+    double weight = 0.0;     // modelled cost, heavy class pairs launch first
+    int4 *d_tiles = nullptr;
+    int *d_tmol = nullptr;
+    long ntiles = 0;
+};
+
+}  // namespace
+
+struct mjk_batch {
+    mjk_basis_prod *bp = nullptr;
+    int n_mol = 0, max_nao = 0;
+    size_t total = 0;
+    std::vector<MolRec> mols;
+    MolRec *d_mols = nullptr;
+    std::vector<int> mol_off;  // [ncptype][n_mol + 1]
+    std::vector<BatchCP> cps;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *g_key[3] = {nullptr, nullptr, nullptr};
+    double g_tol = 0.0, g_pc = 0.0;
+};
+
+namespace {
+
+void batch_drop_graph(mjk_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int default_kc(const BatchCP &c) { return c.kind == 0 ? 64 : 16; }
+
+// Tiles of one class pair over every molecule: bra blocks of c.bra_tile pairs times ket
+// chunks of c.kc pairs, both inside one molecule; same-class pairs keep q <= p.
+int batch_make_tiles(mjk_batch *b, BatchCP &c)
+{
+    if (c.d_tiles) (void)hipFree(c.d_tiles);
+    if (c.d_tmol) (void)hipFree(c.d_tmol);
+    c.d_tiles = nullptr;
+    c.d_tmol = nullptr;
+    c.ntiles = 0;
+    if (c.kind == 2) return MJK_OK;  // computed tiles (single molecule)
+    std::vector<int4> t;
+    std::vector<int> m;
+    const int nm1 = b->n_mol + 1;
+    const int *ob = &b->mol_off[(size_t)c.cb * nm1], *ok = &b->mol_off[(size_t)c.ck * nm1];
+    for (int mol = 0; mol < b->n_mol; ++mol) {
+        const int pb0 = ob[mol], pb1 = ob[mol + 1], pk0 = ok[mol], pk1 = ok[mol + 1];
+        if (pb0 == pb1 || pk0 == pk1) continue;
+        for (int p0 = pb0; p0 < pb1; p0 += c.bra_tile) {
+            const int p1 = std::min(p0 + c.bra_tile, pb1);
+            for (int q0 = pk0; q0 < pk1; q0 += c.kc) {
+                if (c.same && q0 > p1 - 1) break;
+                t.push_back(make_int4(p0, p1, q0, std::min(q0 + c.kc, pk1)));
+                m.push_back(mol);
+            }
+        }
+    }
+    c.ntiles = (long)t.size();
+    if (t.empty()) return MJK_OK;
+    hipError_t e;
+    if ((e = hipMalloc((void **)&c.d_tiles, t.size() * sizeof(int4))) != hipSuccess ||
+        (e = hipMalloc((void **)&c.d_tmol, m.size() * sizeof(int))) != hipSuccess ||
+        (e = hipMemcpy(c.d_tiles, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c.d_tmol, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? MJK_ERR_OOM : MJK_ERR_HIP,
+                    std::string("batch tiles: ") + hipGetErrorString(e));
+    return MJK_OK;
+}
+
+int batch_launch_cp(mjk_batch *b, const BatchCP &c, double *vj, double *vk, const double *dm,
+                    const mjk_screen &sc, hipStream_t st)
+{
+    mjk_basis_prod *bp = b->bp;
+    const ClassType &bra = bp->cp[c.cb], &ket = bp->cp[c.ck];
+    if (bra.npairs == 0 || ket.npairs == 0) return MJK_OK;
+    if (c.kind != 2 && c.ntiles == 0) return MJK_OK;
+    JKArgs a{};
+    a.bra = pair_set(bp, bra);
+    a.ket = pair_set(bp, ket);
+    a.bx = bp->bx; a.by = bp->by; a.bz = bp->bz;
+    a.ao_loc = bp->ao_loc;
+    a.dm = dm; a.vj = vj; a.vk = vk;
+    a.tol = sc.direct_scf_tol;
+    a.prim_tol = sc.direct_scf_tol * 1e-6;
+    a.nbas = bp->nbas;
+    a.nao = b->mols[0].nao;
+    a.n_dm = 1;
+    a.same = c.same;
+    a.counter = bp->counters;
+    a.slots = bp->counters + 2;
+    if (c.kind != 2) {
+        a.tiles = c.d_tiles;
+        a.tile_mol = c.d_tmol;
+        a.mols = b->d_mols;
+        a.ndesc = c.ntiles;
+    }
+    const double pc = sc.precision_cutoff;
+    a.lo = pc > 0 ? pc : 0.0;
+    a.hi = INFINITY;
+    long nt = 0;
+    hipError_t e = launch_class(c.l[0], c.l[1], c.l[2], c.l[3], false, a, st, &nt);
+    if (e != hipSuccess) return hip_fail(e, "mjk_batch_build: class launch");
+    if (pc > 0) {
+        a.lo = 0.0;
+        a.hi = pc;
+        a.counter = bp->counters + 1;
+        e = launch_class(c.l[0], c.l[1], c.l[2], c.l[3], true, a, st, &nt);
+        if (e != hipSuccess) return hip_fail(e, "mjk_batch_build: FP32 class launch");
+    }
+    return MJK_OK;
+}
+
+int batch_enqueue(mjk_batch *b, double *vj, double *vk, const double *dm, const mjk_screen &sc, hipStream_t st)
+{
+    hipError_t e;
+    if (vj && (e = launch_fill(vj, b->total, 0.0, st)) != hipSuccess) return hip_fail(e, "mjk_batch_build: zero vj");
+    if (vk && (e = launch_fill(vk, b->total, 0.0, st)) != hipSuccess) return hip_fail(e, "mjk_batch_build: zero vk");
+    if ((e = launch_fill_u64(b->bp->counters, 4, st)) != hipSuccess) return hip_fail(e, "mjk_batch_build: counters");
+    for (const BatchCP &c : b->cps) {
+        int rc = batch_launch_cp(b, c, vj, vk, dm, sc, st);
+        if (rc) return rc;
+    }
+    if (vj && (e = launch_symmetrize_batch(vj, b->d_mols, b->n_mol, b->max_nao, 2.0, st)) != hipSuccess)
+        return hip_fail(e, "mjk_batch_build: symmetrize vj");
+    if (vk && (e = launch_symmetrize_batch(vk, b->d_mols, b->n_mol, b->max_nao, 1.0, st)) != hipSuccess)
+        return hip_fail(e, "mjk_batch_build: symmetrize vk");
+    return MJK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mjk_batch_create(mjk_batch **pb, mjk_basis_prod **pbp, int n_mol, const int32_t *mol_pair_off,
+                     const int64_t *dm_off, const int32_t *nao, const int32_t *ao_base)
+{
+    if (!pb || !pbp || !*pbp || n_mol <= 0 || !mol_pair_off || !dm_off || !nao || !ao_base)
+        return fail(MJK_ERR_ARG, "mjk_batch_create: bad argument");
+    *pb = nullptr;
+    mjk_basis_prod *bp = *pbp;
+    const int nct = bp->ncptype, nm1 = n_mol + 1;
+    for (int c = 0; c < nct; ++c) {
+        const int32_t *o = mol_pair_off + (size_t)c * nm1;
+        if (o[0] != 0 || o[n_mol] != bp->cp[c].npairs)
+            return fail(MJK_ERR_ARG, "mjk_batch_create: molecule pair offsets do not cover class " + std::to_string(c));
+        for (int m = 0; m < n_mol; ++m)
+            if (o[m + 1] < o[m]) return fail(MJK_ERR_ARG, "mjk_batch_create: molecule pair offsets decrease");
+    }
+    mjk_batch *b = new mjk_batch();
+    b->n_mol = n_mol;
+    b->mol_off.assign(mol_pair_off, mol_pair_off + (size_t)nct * nm1);
+    for (int m = 0; m < n_mol; ++m) {
+        if (nao[m] <= 0 || dm_off[m] < 0 || ao_base[m] < 0) {
+            delete b;
+            return fail(MJK_ERR_ARG, "mjk_batch_create: bad molecule record " + std::to_string(m));
+        }
+        b->mols.push_back(MolRec{(long)dm_off[m], nao[m], ao_base[m]});
+        b->max_nao = std::max(b->max_nao, (int)nao[m]);
+        b->total = std::max(b->total, (size_t)dm_off[m] + (size_t)nao[m] * nao[m]);
+    }
+    hipError_t e;
+    if ((e = hipMalloc((void **)&b->d_mols, n_mol * sizeof(MolRec))) != hipSuccess ||
+        (e = hipMemcpy(b->d_mols, b->mols.data(), n_mol * sizeof(MolRec), hipMemcpyHostToDevice)) != hipSuccess) {
+        if (b->d_mols) (void)hipFree(b->d_mols);
+        delete b;
+        return hip_fail(e, "mjk_batch_create");
+    }
+    b->bp = bp;
+    for (int i = 0; i < nct; ++i)
+        for (int j = 0; j <= i; ++j) {
+            BatchCP c;
+            const ClassType *c1 = &bp->cp[i], *c2 = &bp->cp[j];
+            c.cb = i;
+            c.ck = j;
+            if (ncart_h(c1->l_bra) * ncart_h(c1->l_ket) < ncart_h(c2->l_bra) * ncart_h(c2->l_ket)) {
+                std::swap(c1, c2);
+                std::swap(c.cb, c.ck);
+            }
+            c.l[0] = c1->l_bra; c.l[1] = c1->l_ket; c.l[2] = c2->l_bra; c.l[3] = c2->l_ket;
+            c.same = i == j;
+            c.kind = class_kernel_info(c.l[0], c.l[1], c.l[2], c.l[3], &c.bra_tile);
+            if (c.kind < 0 || (c.kind == 2 && n_mol > 1)) {
+                *pbp = bp;  // ownership stays with the caller
+                b->bp = nullptr;
+                mjk_batch_destroy(&b);
+                return fail(MJK_ERR_UNSUPPORTED, std::string("mjk_batch_create: class (") + "spdf"[c.l[0]] +
+                                                     "spdf"[c.l[1]] + "|" + "spdf"[c.l[2]] + "spdf"[c.l[3]] +
+                                                     ") has no batched kernel");
+            }
+            c.kc = default_kc(c);
+            c.weight = (double)c1->npairs * c2->npairs * c1->nprim12 * c2->nprim12 * ncart_h(c.l[0]) *
+                       ncart_h(c.l[1]) * ncart_h(c.l[2]) * ncart_h(c.l[3]);
+            b->cps.push_back(c);
+        }
+    std::sort(b->cps.begin(), b->cps.end(), [](const BatchCP &x, const BatchCP &y) { return x.weight > y.weight; });
+    for (BatchCP &c : b->cps) {
+        int rc = batch_make_tiles(b, c);
+        if (rc) {
+            *pbp = bp;
+            b->bp = nullptr;
+            mjk_batch_destroy(&b);
+            return rc;
+        }
+    }
+    *pbp = nullptr;  // the batch owns the basis-product cache from here on
+    *pb = b;
+    return MJK_OK;
+}
+
+int mjk_batch_destroy(mjk_batch **pb)
+{
+    if (!pb || !*pb) return MJK_OK;
+    mjk_batch *b = *pb;
+    batch_drop_graph(b);
+    for (BatchCP &c : b->cps) {
+        if (c.d_tiles) (void)hipFree(c.d_tiles);
+        if (c.d_tmol) (void)hipFree(c.d_tmol);
+    }
+    if (b->d_mols) (void)hipFree(b->d_mols);
+    if (b->bp) mjk_del_basis_prod(&b->bp);
+    delete b;
+    *pb = nullptr;
+    return MJK_OK;
+}
+
+int mjk_batch_build(mjk_batch *b, double *vj, double *vk, const double *dm, const mjk_screen *screen,
+                    int use_graph, void *stream)
+{
+    if (!b || !dm || !screen) return fail(MJK_ERR_ARG, "mjk_batch_build: bad argument");
+    if (screen->density_screen) return fail(MJK_ERR_ARG, "mjk_batch_build: batched builds screen by Schwarz only");
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != b->bp->device) return fail(MJK_ERR_ARG, "mjk_batch_build: current device differs from the batch's");
+    if (!vj && !vk) return MJK_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (!use_graph) return batch_enqueue(b, vj, vk, dm, *screen, st);
+    if (!st) return fail(MJK_ERR_ARG, "mjk_batch_build: graph capture needs a non-default stream");
+    const bool same_key = b->exec && b->g_key[0] == vj && b->g_key[1] == vk && b->g_key[2] == dm &&
+                          b->g_tol == screen->direct_scf_tol && b->g_pc == screen->precision_cutoff;
+    if (!same_key) {
+        batch_drop_graph(b);
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(e, "mjk_batch_build: begin capture");
+        int rc = batch_enqueue(b, vj, vk, dm, *screen, st);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "mjk_batch_build: end capture");
+        b->graph = g;
+        if ((e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0)) != hipSuccess) {
+            batch_drop_graph(b);
+            return hip_fail(e, "mjk_batch_build: instantiate");
+        }
+        b->g_key[0] = vj;
+        b->g_key[1] = vk;
+        b->g_key[2] = dm;
+        b->g_tol = screen->direct_scf_tol;
+        b->g_pc = screen->precision_cutoff;
+    }
+    hipError_t e = hipGraphLaunch(b->exec, st);
+    if (e != hipSuccess) return hip_fail(e, "mjk_batch_build: graph launch");
+    return MJK_OK;
+}
+
+int mjk_batch_num_class_pairs(const mjk_batch *b) { return b ? (int)b->cps.size() : 0; }
+
+int mjk_batch_tiles(const mjk_batch *b, int32_t *out, int n)
+{
+    if (!b || !out || n < (int)b->cps.size()) return fail(MJK_ERR_ARG, "mjk_batch_tiles: bad argument");
+    for (size_t i = 0; i < b->cps.size(); ++i) {
+        const BatchCP &c = b->cps[i];
+        int32_t *o = out + 10 * i;
+        o[0] = c.cb; o[1] = c.ck;
+        o[2] = c.l[0]; o[3] = c.l[1]; o[4] = c.l[2]; o[5] = c.l[3];
+        o[6] = c.kc;
+        o[7] = (int32_t)std::min<long>(c.ntiles, 0x7fffffff);
+        o[8] = c.kind;
+        o[9] = c.bra_tile;
+    }
+    return MJK_OK;
+}
+
+int mjk_batch_set_kc(mjk_batch *b, const int32_t *kc, int n)
+{
+    if (!b || !kc || n != (int)b->cps.size()) return fail(MJK_ERR_ARG, "mjk_batch_set_kc: bad argument");
+    for (int i = 0; i < n; ++i)
+        if (kc[i] < 1 || kc[i] > 4096) return fail(MJK_ERR_ARG, "mjk_batch_set_kc: ket chunk out of range");
+    batch_drop_graph(b);
+    for (int i = 0; i < n; ++i) {
+        if (b->cps[i].kc == kc[i]) continue;
+        b->cps[i].kc = kc[i];
+        int rc = batch_make_tiles(b, b->cps[i]);
+        if (rc) return rc;
+    }
+    return MJK_OK;
+}
+
+int mjk_batch_time_class_pair(mjk_batch *b, int i, double *vj, double *vk, const double *dm,
+                              const mjk_screen *screen, int reps, void *stream, float *ms)
+{
+    if (!b || i < 0 || i >= (int)b->cps.size() || !dm || !screen || reps < 1 || !ms)
+        return fail(MJK_ERR_ARG, "mjk_batch_time_class_pair: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    hipError_t e;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_fail(e, "event");
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(e, "event");
+    }
+    int rc = batch_launch_cp(b, b->cps[i], vj, vk, dm, *screen, st);  // warm-up
+    if (!rc) (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps && !rc; ++r) rc = batch_launch_cp(b, b->cps[i], vj, vk, dm, *screen, st);
+    if (!rc) {
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        *ms = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int mjk_batch_stats(mjk_batch *b, double *out)
+{
+    if (!b) return fail(MJK_ERR_ARG, "mjk_batch_stats: null batch");
+    return mjk_last_stats(b->bp, out);
 }
 
 int mjk_last_stats(mjk_basis_prod *bp, double *out)

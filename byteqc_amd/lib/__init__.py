@@ -30,7 +30,10 @@ class Screen(ctypes.Structure):
 
 EXPORTS = ('mjk_version', 'mjk_last_error', 'mjk_q_cond', 'mjk_init_basis_prod',
            'mjk_del_basis_prod', 'mjk_nao', 'mjk_build_jk', 'mjk_build_jk_all',
-           'mjk_last_stats', 'mjk_int1e')
+           'mjk_last_stats', 'mjk_int1e', 'mjk_gemm_bf16', 'mjk_bias_act', 'mjk_act_backward',
+           'mjk_mse_grad', 'mjk_adam', 'mjk_to_bf16', 'mjk_batch_create', 'mjk_batch_destroy',
+           'mjk_batch_build', 'mjk_batch_num_class_pairs', 'mjk_batch_tiles', 'mjk_batch_set_kc',
+           'mjk_batch_time_class_pair', 'mjk_batch_stats')
 
 _lib = None
 
@@ -59,6 +62,23 @@ def load():
     lib.mjk_build_jk_all.argtypes = [p, p, p, p, i, i, ctypes.POINTER(Screen), p]
     lib.mjk_last_stats.argtypes = [p, p]
     lib.mjk_int1e.argtypes = [p, i, p, i, p, p, p, p, i]
+    f, sz = ctypes.c_float, ctypes.c_size_t
+    lib.mjk_gemm_bf16.argtypes = [i, i, i, i, p, i, p, i, p, i, f, p]
+    lib.mjk_bias_act.argtypes = [i, i, p, p, p, i, p]
+    lib.mjk_act_backward.argtypes = [i, i, p, p, p, p, i, p]
+    lib.mjk_mse_grad.argtypes = [sz, p, p, p, f, p, p, p]
+    lib.mjk_adam.argtypes = [sz, p, p, p, p, p, f, f, f, f, i, p]
+    lib.mjk_to_bf16.argtypes = [sz, p, p, p]
+    pp = ctypes.POINTER(p)
+    lib.mjk_batch_create.argtypes = [pp, pp, i, p, p, p, p]
+    lib.mjk_batch_destroy.argtypes = [pp]
+    lib.mjk_batch_build.argtypes = [p, p, p, p, ctypes.POINTER(Screen), i, p]
+    lib.mjk_batch_num_class_pairs.argtypes = [p]
+    lib.mjk_batch_tiles.argtypes = [p, p, i]
+    lib.mjk_batch_set_kc.argtypes = [p, p, i]
+    lib.mjk_batch_time_class_pair.argtypes = [p, i, p, p, p, ctypes.POINTER(Screen), i, p,
+                                              ctypes.POINTER(ctypes.c_float)]
+    lib.mjk_batch_stats.argtypes = [p, p]
     _lib = lib
     return lib
 

@@ -198,6 +198,9 @@ class RHF:
     def get_init_guess(self, key='sad'):
         if key in ('sad', 'atom', 'minao'):
             return init_guess_by_atom(self.mol)
+        if key == 'learned':
+            from ..guess.learned import learned_guess
+            return learned_guess(self.mol, self._s)
         if key in ('1e', 'hcore'):
             torch = _torch()
             e, c = eig(torch.as_tensor(self._h, device='cuda'), torch.as_tensor(self._s, device='cuda'))

@@ -109,6 +109,70 @@ int mjk_last_stats(mjk_basis_prod *bp, double *out);
 int mjk_int1e(const int32_t *atm, int natm, const int32_t *bas, int nbas, const double *env,
               double *s, double *t, double *v, int nthreads);
 
+/* ---- Batched builds: many molecules, one launch per class pair ------------------------
+ * No counterpart in the reference, whose get_jk handles one molecule per call and whose
+ * multi-GPU map hands whole class pairs to devices (cuobc/lib/multigpu.py:264-291).
+ *
+ * The caller builds ONE basis-product cache over all molecules (shells and atoms
+ * concatenated; only intra-molecule shell pairs listed; inside each class the pairs of
+ * molecule 0 first, then molecule 1, ...) and hands it to mjk_batch_create, which takes
+ * ownership (*pbp is set to NULL on success).  mol_pair_off is [ncptype][n_mol + 1]: the
+ * pairs of molecule m in class c are [off[c][m], off[c][m+1]) of that class.  Molecule m's
+ * D, J and K are nao[m] x nao[m] row-major at element dm_off[m] of the batch arrays, and
+ * its shells' Cartesian AO indices start at ao_base[m] of the concatenated basis.  Every
+ * quartet class launches once for all molecules over a host-built table of tiles. */
+typedef struct mjk_batch mjk_batch;
+int mjk_batch_create(mjk_batch **pb, mjk_basis_prod **pbp, int n_mol, const int32_t *mol_pair_off,
+                     const int64_t *dm_off, const int32_t *nao, const int32_t *ao_base);
+int mjk_batch_destroy(mjk_batch **pb);
+/* J = 2(V + V^T), K = V + V^T per molecule (vj or vk may be NULL); Schwarz screening only
+ * (screen->density_screen must be 0).  use_graph != 0 records the whole build into a
+ * hipGraph on the first call and replays it while (vj, vk, dm, tolerances) stay the same;
+ * it needs a non-default stream.  The graph holds no memset nodes. */
+int mjk_batch_build(mjk_batch *b, double *vj, double *vk, const double *dm, const mjk_screen *screen,
+                    int use_graph, void *stream);
+int mjk_batch_num_class_pairs(const mjk_batch *b);
+/* Per class pair, 10 int32 each: bra class, ket class, l_a, l_b, l_c, l_d, ket chunk, tiles,
+ * kernel family (0 register, 1 group, 2 block-LDS) and bra pairs per tile. */
+int mjk_batch_tiles(const mjk_batch *b, int32_t *out, int n);
+/* Set the ket chunk (ket pairs per tile) of every class pair, in mjk_batch_tiles order. */
+int mjk_batch_set_kc(mjk_batch *b, const int32_t *kc, int n);
+/* Average time (ms, hipEvents) of class pair i over reps launches after one warm-up; the
+ * launches accumulate into vj/vk, so the caller rebuilds afterwards.  The autotuner's probe. */
+int mjk_batch_time_class_pair(mjk_batch *b, int i, double *vj, double *vk, const double *dm,
+                              const mjk_screen *screen, int reps, void *stream, float *ms);
+/* Counters of the last build, as mjk_last_stats. */
+int mjk_batch_stats(mjk_batch *b, double *out);
+
+/* ---- Learned initial guess: perceptron kernels ------------------------------------
+ * No counterpart in the reference, whose SCF starts from PySCF's minao/SAD guess
+ * (PySCF RHF.get_init_guess, inherited at cuobc/scf/hf.py:272-274).  All pointers are
+ * device pointers; bf16 values are uint16_t bit patterns (round to nearest even); every
+ * call is asynchronous on `stream`.  Matrices are row-major.
+ *
+ * c[m][n] = op(a) op(b) + beta c, bf16 operands, fp32 accumulate on MFMA:
+ *   layout 0: a is [m][k] (lda), b is [k][n] (ldb)
+ *   layout 1: a is [m][k],       b is [n][k]        (c = a b^T)
+ *   layout 2: a is [k][m],       b is [k][n]        (c = a^T b) */
+int mjk_gemm_bf16(int layout, int m, int n, int k, const uint16_t *a, int lda,
+                  const uint16_t *b, int ldb, float *c, int ldc, float beta, void *stream);
+/* z[m][n] += bias[n] (bias may be NULL); a_out = bf16(act(z)) if a_out is not NULL;
+ * act 0 = identity, 1 = SiLU z / (1 + exp(-z)). */
+int mjk_bias_act(int m, int n, float *z, const float *bias, uint16_t *a_out, int act,
+                 void *stream);
+/* dz = bf16(da * act'(z)); dbias[n] = sum_m da act'(z) in fp32 when dbias is not NULL. */
+int mjk_act_backward(int m, int n, const float *da, const float *z, uint16_t *dz, float *dbias,
+                     int act, void *stream);
+/* dy = scale * mask * (y - target) over n elements (mask may be NULL = 1); if loss is not
+ * NULL, *loss += sum mask (y - target)^2 (the caller zeroes it). */
+int mjk_mse_grad(size_t n, const float *y, const float *target, const float *mask, float scale,
+                 float *dy, double *loss, void *stream);
+/* One Adam step (step >= 1, bias-corrected) on n fp32 weights; w_bf16 (may be NULL)
+ * receives the rounded copy the GEMMs read. */
+int mjk_adam(size_t n, float *w, const float *g, float *m1, float *m2, uint16_t *w_bf16,
+             float lr, float beta1, float beta2, float eps, int step, void *stream);
+int mjk_to_bf16(size_t n, const float *x, uint16_t *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,3 +56,18 @@ def test_check_raises():
         mjk.check(mjk.MJK_ERR_ARG, 'x')
     with pytest.raises(NotImplementedError):
         mjk.check(mjk.MJK_ERR_UNSUPPORTED, 'x')
+
+
+def test_perceptron_entry_points_reject_bad_arguments():
+    """Argument checks of the learned-guess kernels return codes before any launch."""
+    lib = mjk.load()
+    assert lib.mjk_gemm_bf16(3, 4, 4, 4, 8, 4, 8, 4, 8, 4, 0.0, None) == mjk.MJK_ERR_ARG
+    assert b'layout' in lib.mjk_last_error()
+    assert lib.mjk_gemm_bf16(0, 4, 4, 4, 8, 2, 8, 4, 8, 4, 0.0, None) == mjk.MJK_ERR_ARG
+    assert b'leading dimension' in lib.mjk_last_error()
+    assert lib.mjk_gemm_bf16(0, 4, 4, 4, None, 4, None, 4, 8, 4, 0.0, None) == mjk.MJK_ERR_ARG
+    assert lib.mjk_bias_act(4, 4, None, None, None, 1, None) == mjk.MJK_ERR_ARG
+    assert lib.mjk_act_backward(4, 4, 8, None, 8, None, 1, None) == mjk.MJK_ERR_ARG
+    assert lib.mjk_adam(4, 8, 8, 8, 8, None, 1e-3, 0.9, 0.999, 1e-8, 0, None) == mjk.MJK_ERR_ARG
+    assert lib.mjk_mse_grad(4, None, 8, None, 1.0, 8, None, None) == mjk.MJK_ERR_ARG
+    assert lib.mjk_to_bf16(4, None, 8, None) == mjk.MJK_ERR_ARG
