@@ -74,13 +74,14 @@ jk_grp_kernel(const JKArgs a)
     const int g = lane / S, sub = lane % S;
     const bool lane_ok = g < W;
     const int gg = lane_ok ? g : 0;
-    int p, q_begin, q_end;
+    int p, q_begin, q_end, p_first = 0;
     bool pvalid;
     const double *D = a.dm;
     double *VJ = a.vj, *VK = a.vk;
     int nao = a.nao, ao_base = 0;
     if (a.tiles) {
         const int4 d = a.tiles[tile];
+        p_first = d.x;
         p = d.x + g;
         pvalid = lane_ok && p < d.y;
         q_begin = d.z;
@@ -100,7 +101,9 @@ jk_grp_kernel(const JKArgs a)
         if (q_end > a.ket.npairs) q_end = a.ket.npairs;
         pvalid = lane_ok && p < a.bra.npairs;
     }
-    const int pp = pvalid ? p : 0;
+    // lanes without a bra pair borrow the tile's first one, which lies in the same molecule,
+    // so every index they form stays inside this molecule's D block
+    const int pp = pvalid ? p : p_first;
     const int4 b4 = a.bra.pint[pp];
     const double *bd = a.bra.pdbl + 8 * pp;
     const int ish = b4.x, jsh = b4.y;

@@ -20,6 +20,25 @@
 namespace mjk {
 
 constexpr int REG_BRA = 256;  // bra pairs (lanes) per tile
+// Workgroups per CU each class is compiled for.  Two halve the registers a lane may hold:
+// faster where the extra wave hides more latency than the spills cost, measured per class
+// on (H2O)64/def2-TZVP ((fp|ss) 92 -> 70 ms, (ds|pp) 95 -> 86, (dp|ss) 68 -> 60, while
+// (dp|ps) went 80 -> 112 and stays at one).
+constexpr int reg_min_blocks(int la, int lb, int lc, int ld)
+{
+    const int key = ((la * 4 + lb) * 4 + lc) * 4 + ld;
+    switch (key) {
+    case ((3 * 4 + 1) * 4 + 0) * 4 + 0:  // (fp|ss)
+    case ((3 * 4 + 0) * 4 + 1) * 4 + 0:  // (fs|ps)
+    case ((2 * 4 + 0) * 4 + 1) * 4 + 1:  // (ds|pp)
+    case ((2 * 4 + 1) * 4 + 0) * 4 + 0:  // (dp|ss)
+    case ((2 * 4 + 0) * 4 + 1) * 4 + 0:  // (ds|ps)
+    case ((1 * 4 + 1) * 4 + 1) * 4 + 0:  // (pp|ps)
+        return 2;
+    default:
+        return 1;
+    }
+}
 constexpr int REG_KC = 128;   // longest ket chunk per tile
 
 // Tiles of a same-class launch with ket chunks of kc pairs (kc divides REG_BRA): bra block
@@ -53,7 +72,7 @@ __device__ inline double wave_sum(double v)
 }
 
 template <typename R, int LA, int LB, int LC, int LD>
-__global__ void __launch_bounds__(REG_BRA)
+__global__ void __launch_bounds__(REG_BRA, reg_min_blocks(LA, LB, LC, LD))
 jk_reg_kernel(const JKArgs a)
 {
     constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
@@ -61,18 +80,20 @@ jk_reg_kernel(const JKArgs a)
     constexpr int NFAB = NFA * NFB, NFCD = NFC * NFD;
     constexpr int G2 = (LC + 1) * (LD + 1);
     constexpr int GS = (LA + 1) * (LB + 1) * G2;
+    constexpr bool D_EARLY = (NFA + NFB) * (NFC + NFD) + NFCD <= 16;
 
     // ---- tile
     const long tile = a.tile_base + (long)blockIdx.x;
     const int t = threadIdx.x;
     const int lane = t & 63;
-    int p, q_begin, q_end;
+    int p, q_begin, q_end, p_first = 0;
     bool pvalid;
     const double *__restrict__ D = a.dm;
     double *VJ = a.vj, *VK = a.vk;
     int nao = a.nao, ao_base = 0;
     if (a.tiles) {
         const int4 d = a.tiles[tile];
+        p_first = d.x;
         p = d.x + t;
         pvalid = p < d.y;
         q_begin = d.z;
@@ -104,7 +125,9 @@ jk_reg_kernel(const JKArgs a)
         if (q_end > a.ket.npairs) q_end = a.ket.npairs;
     }
 
-    const int pp = pvalid ? p : 0;
+    // lanes without a bra pair borrow the tile's first one, which lies in the same molecule,
+    // so every index they form stays inside this molecule's D block
+    const int pp = pvalid ? p : p_first;
     const int4 b4 = a.bra.pint[pp];
     const double *bd = a.bra.pdbl + 8 * pp;
     const int ish = b4.x, jsh = b4.y;
@@ -139,6 +162,20 @@ jk_reg_kernel(const JKArgs a)
         seg_head = 63 - __builtin_clzll(below);
         seg_tail = tail;
     }
+    // lanes of a wave holding the same shell j share one LDS slot for K_jl (the slot of the
+    // first such lane, which alone issues the global atomics): one atomic per distinct j
+    // instead of one per lane
+    constexpr int NJL = NFB * NFD;
+    __shared__ double kjl_slot[REG_BRA / 64][64][NJL];
+    const int wv = t >> 6;
+    int jlead = lane;
+    {
+        const int key = pvalid ? jsh : -1 - lane;
+        for (int u = 63; u >= 0; --u)
+            if (__shfl(key, u, 64) == key) jlead = u;
+#pragma unroll
+        for (int e = 0; e < NJL; ++e) kjl_slot[wv][lane][e] = 0.0;
+    }
     const double A[3] = {bd[1], bd[2], bd[3]};
     const double AB[3] = {bd[4], bd[5], bd[6]};
     const int i0 = b4.z - ao_base, j0 = b4.w - ao_base;
@@ -147,7 +184,7 @@ jk_reg_kernel(const JKArgs a)
 
     double dij[NFAB];
 #pragma unroll
-    for (int r = 0; r < NFAB; ++r) dij[r] = doj ? D[ri + (r / NFB) * nao + j0 + r % NFB] : 0.0;
+    for (int r = 0; r < NFAB; ++r) dij[r] = (doj && pvalid) ? D[ri + (r / NFB) * nao + j0 + r % NFB] : 0.0;
     double jij[NFAB];
     double kik[NFA][NFC], kjk[NFB][NFC];
 #pragma unroll
@@ -176,24 +213,28 @@ jk_reg_kernel(const JKArgs a)
         const double qq = kd[0];
         const double Cc[3] = {kd[1], kd[2], kd[3]};
         const double CD[3] = {kd[4], kd[5], kd[6]};
-        // ---- D elements of this quartet, issued before the integrals
+        // ---- D elements of this quartet: issued before the integrals when they are few,
+        // else next to the digestion (the 2D tables are dead there)
         double dkl[NFCD], dik[NFA][NFC], djk[NFB][NFC], dil[NFA][NFD], djl[NFB][NFD];
+        auto load_d = [&]() {
 #pragma unroll
-        for (int c = 0; c < NFCD; ++c) dkl[c] = doj ? D[(k0 + c / NFD) * nao + l0 + c % NFD] : 0.0;
+            for (int c = 0; c < NFCD; ++c) dkl[c] = doj ? D[(k0 + c / NFD) * nao + l0 + c % NFD] : 0.0;
 #pragma unroll
-        for (int x = 0; x < NFA; ++x) {
+            for (int x = 0; x < NFA; ++x) {
 #pragma unroll
-            for (int c = 0; c < NFC; ++c) dik[x][c] = dok ? D[ri + x * nao + k0 + c] : 0.0;
+                for (int c = 0; c < NFC; ++c) dik[x][c] = (dok && pvalid) ? D[ri + x * nao + k0 + c] : 0.0;
 #pragma unroll
-            for (int d = 0; d < NFD; ++d) dil[x][d] = dok ? D[ri + x * nao + l0 + d] : 0.0;
-        }
+                for (int d = 0; d < NFD; ++d) dil[x][d] = (dok && pvalid) ? D[ri + x * nao + l0 + d] : 0.0;
+            }
 #pragma unroll
-        for (int x = 0; x < NFB; ++x) {
+            for (int x = 0; x < NFB; ++x) {
 #pragma unroll
-            for (int c = 0; c < NFC; ++c) djk[x][c] = dok ? D[rj + x * nao + k0 + c] : 0.0;
+                for (int c = 0; c < NFC; ++c) djk[x][c] = (dok && pvalid) ? D[rj + x * nao + k0 + c] : 0.0;
 #pragma unroll
-            for (int d = 0; d < NFD; ++d) djl[x][d] = dok ? D[rj + x * nao + l0 + d] : 0.0;
-        }
+                for (int d = 0; d < NFD; ++d) djl[x][d] = (dok && pvalid) ? D[rj + x * nao + l0 + d] : 0.0;
+            }
+        };
+        if constexpr (D_EARLY) load_d();
         if (ksh != cur_k) {
             if (dok && kdirty) {
 #pragma unroll
@@ -289,6 +330,7 @@ jk_reg_kernel(const JKArgs a)
                 }
             }
         }
+        if constexpr (!D_EARLY) load_d();
         // ---- digestion (inactive lanes hold zeros)
         if (doj) {
             double jkl[NFCD];
@@ -343,7 +385,23 @@ jk_reg_kernel(const JKArgs a)
 #pragma unroll
                 for (int x = 0; x < NFB; ++x)
 #pragma unroll
-                    for (int d = 0; d < NFD; ++d) unsafeAtomicAdd(VK + rj + x * nao + l0 + d, kjl[x][d]);
+                    for (int d = 0; d < NFD; ++d) atomicAdd(&kjl_slot[wv][jlead][x * NFD + d], kjl[x][d]);
+            }
+            // the wave's LDS adds complete before its leaders read the slots
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (jlead == lane && pvalid) {
+#pragma unroll
+                for (int x = 0; x < NFB; ++x)
+#pragma unroll
+                    for (int d = 0; d < NFD; ++d) {
+                        const double v = kjl_slot[wv][lane][x * NFD + d];
+                        if (v != 0.0) {
+                            unsafeAtomicAdd(VK + rj + x * nao + l0 + d, v);
+                            kjl_slot[wv][lane][x * NFD + d] = 0.0;
+                        }
+                    }
             }
         }
     }

@@ -30,8 +30,11 @@ def _torch():
     return torch
 
 
-def _label(l, np_):
-    return f'{"spdf"[l[0]]}{"spdf"[l[1]]}{np_[0]}|{"spdf"[l[2]]}{"spdf"[l[3]]}{np_[1]}'
+def _label(gb, gk):
+    """'(l nprim, l nprim | l nprim, l nprim)' of a bra and a ket pair class, e.g. 's6s3|p3s1'."""
+    def one(g):
+        return f'{"spdf"[g[0][0]]}{g[0][1]}{"spdf"[g[1][0]]}{g[1][1]}'
+    return f'{one(gb)}|{one(gk)}'
 
 
 class BatchJK:
@@ -202,9 +205,8 @@ class BatchJK:
         t = self.tiles()
         keys = []
         for row in t:
-            cb, ck = int(row[0]), int(row[1])
-            gb, gk = self._class_groups(cb), self._class_groups(ck)
-            keys.append(_label(row[2:6], (gb[0][1] * gb[1][1], gk[0][1] * gk[1][1])))
+            keys.append(_label(self._class_groups(int(row[0])), self._class_groups(int(row[1]))))
+        assert len(set(keys)) == len(keys)
         return keys
 
     def _class_groups(self, c):

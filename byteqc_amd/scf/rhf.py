@@ -261,14 +261,15 @@ class RHF:
                 vj, vk = self.get_jk(mol, dm)
                 since_full = 0
             torch.cuda.synchronize()
-            t_jk += time.perf_counter() - tj
+            dt_jk = time.perf_counter() - tj
+            t_jk += dt_jk
             f = h + vj - 0.5 * vk
             e_tot = float(0.5 * torch.sum(dm * (h + f))) + enuc
             err = f @ dm @ s - s @ dm @ f
             gnorm = float(torch.abs(err).max())
             de = None if e_last is None else e_tot - e_last
             self.history.append({'cycle': cycle, 'e_tot': e_tot, 'delta_e': de, 'grad': gnorm,
-                                 't': time.perf_counter() - t_start})
+                                 't': time.perf_counter() - t_start, 't_jk': dt_jk})
             if self.verbose:
                 print(f'cycle {cycle}: E = {e_tot:.12f}  dE = {de}  |[F,D]| = {gnorm:.2e}', flush=True)
             if de is not None and abs(de) < self.conv_tol and gnorm < self.conv_tol_grad:
