@@ -16,7 +16,9 @@ Also reported:
   cpu_baseline  the CPU oracle's Rys J/K (C, OpenMP, all host cores) on the same shells,
                 screening and density, timed on a systematic sample of bra-pair rows and
                 scaled to the full build
-  scf           (--scf) RHF time-to-1e-8 Eh from the SAD guess
+  scf           (--scf) RHF time-to-1e-8 Eh: FP64 + SAD guess (the reference's path) against
+                FP64 + learned guess, mixed (FP32 quartets below a 1e-7 bound, 1e-10
+                screening until max|[F,D]| < 1e-3) + SAD, and mixed + learned
   batch         (--batch N) N seeded water clusters of 1-32 molecules, 6-31G*, random PSD
                 densities (D = 2 Q Q^T, trace = electron count): all N J/K builds as a
                 per-molecule loop of mjk_build_jk_all, as one batched build, as a replayed
@@ -183,16 +185,22 @@ def cpu_baseline(vhfopt, dm_cart_host, target_s):
                       f'shell pairs each, {t_all:.1f} s measured), scaled by {stride}'}
 
 
-def run_scf(mol, precision_cutoff=0.0):
-    """RHF from the SAD guess to |dE| < 1e-8 Eh; time from the guess to convergence."""
-    from byteqc_amd.scf.rhf import RHF
-    mf = RHF(mol, conv_tol=1e-8, precision_cutoff=precision_cutoff)
+def run_scf(mol, precision_cutoff=0.0, guess='sad', early_tol=None):
+    """RHF to |dE| < 1e-8 Eh and max|[F,D]| < 1e-5; wall time from the guess to convergence
+    (guess included; the SAD atomic densities are cached beforehand, as for a warm code)."""
+    from byteqc_amd.scf.rhf import RHF, init_guess_by_atom
+    init_guess_by_atom(mol)  # warm the atomic-density cache
+    mf = RHF(mol, conv_tol=1e-8, precision_cutoff=precision_cutoff, early_tol=early_tol)
     mf.build()
-    e = mf.kernel()
-    return {'e_tot': e, 'converged': mf.converged, 'cycles': mf.cycles,
+    e = mf.kernel(init_guess=guess)
+    return {'e_tot': e, 'converged': mf.converged, 'cycles': mf.cycles, 'guess': guess,
+            'wall_s': mf.timings['guess'] + mf.timings['scf'],
             'time_s': mf.timings['scf'], 'guess_s': mf.timings['guess'], 'jk_s': mf.timings['jk'],
             'setup_s': mf.timings['vhfopt'] + mf.timings['int1e'],
-            'precision_cutoff': precision_cutoff}
+            'precision_cutoff': precision_cutoff, 'early_tol': early_tol,
+            'cycle_jk_s': [round(h['t_jk'], 4) for h in mf.history],
+            'cycle_e': [h['e_tot'] for h in mf.history],
+            'cycle_grad': [h['grad'] for h in mf.history]}
 
 
 def main(argv=None):
@@ -207,6 +215,8 @@ def main(argv=None):
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-classes', action='store_true')
     ap.add_argument('--scf', action='store_true')
+    ap.add_argument('--scf-runs', default='fp64_sad,fp64_learned,mixed_sad,mixed_learned',
+                    help='comma-separated subset of the SCF comparison (fp64_sad always runs)')
     ap.add_argument('--classes-out', default=None, help='write every class timing to this JSON file')
     ap.add_argument('--batch', type=int, default=0, help='also time N batched 6-31G* cluster builds')
     args = ap.parse_args(argv)
@@ -320,13 +330,26 @@ def main(argv=None):
     if args.scf:
         vhfopt.clear()
         torch.cuda.empty_cache()
-        result['scf'] = {'fp64_sad': run_scf(mol)}
-        log(f'scf fp64 {result["scf"]["fp64_sad"]}')
-        result['scf']['mixed_sad'] = run_scf(mol, precision_cutoff=1e-7)
-        r = result['scf']
-        r['mixed_delta_e'] = r['mixed_sad']['e_tot'] - r['fp64_sad']['e_tot']
-        r['speedup'] = r['fp64_sad']['time_s'] / r['mixed_sad']['time_s']
-        log(f'scf mixed {r["mixed_sad"]}  dE={r["mixed_delta_e"]:.2e} speedup={r["speedup"]:.2f}')
+        from byteqc_amd.guess.learned import model_path
+        learned = 'learned' if os.path.exists(model_path(mol.basis)) else 'sad'
+        runs = {'fp64_sad': dict(),
+                'fp64_learned': dict(guess=learned),
+                'mixed_sad': dict(precision_cutoff=1e-7, early_tol=1e-10),
+                'mixed_learned': dict(precision_cutoff=1e-7, early_tol=1e-10, guess=learned)}
+        wanted = set(args.scf_runs.split(',')) | {'fp64_sad'}
+        runs = {k: v for k, v in runs.items() if k in wanted}
+        r = result['scf'] = {}
+        for name, kw in runs.items():
+            r[name] = run_scf(mol, **kw)
+            log(f'scf {name}: {r[name]["cycles"]} cycles, {r[name]["wall_s"]:.2f} s '
+                f'(J/K {r[name]["jk_s"]:.2f} s), E = {r[name]["e_tot"]:.10f}')
+        base = r['fp64_sad']
+        for name in runs:
+            r[name]['delta_e_vs_fp64_sad'] = r[name]['e_tot'] - base['e_tot']
+            r[name]['speedup_vs_fp64_sad'] = base['wall_s'] / r[name]['wall_s']
+        if 'mixed_learned' in r:
+            log(f'scf mixed+learned vs fp64+SAD: {r["mixed_learned"]["speedup_vs_fp64_sad"]:.2f}x, '
+                f'dE = {r["mixed_learned"]["delta_e_vs_fp64_sad"]:.1e}')
     if args.batch:
         vhfopt.clear()
         torch.cuda.empty_cache()

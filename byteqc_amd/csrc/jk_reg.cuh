@@ -202,14 +202,30 @@ jk_reg_kernel(const JKArgs a)
     unsigned long long nquart = 0, nslots = 0;
     const int npb = a.bra.nprim, npk = a.ket.nprim;
 
+    // the next ket's records are loaded one iteration ahead (software pipelining), so their
+    // latency overlaps the current quartet
+    int4 k4n = make_int4(0, 0, 0, 0);
+    double kdn[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (q_begin < q_end) {
+        k4n = a.ket.pint[q_begin];
+#pragma unroll
+        for (int e = 0; e < 7; ++e) kdn[e] = a.ket.pdbl[8 * q_begin + e];
+    }
     for (int q = q_begin; q < q_end; ++q) {
-        // ---- ket q: all uniform, one level of loads
-        const int4 k4 = a.ket.pint[q];
+        // ---- ket q: all uniform
+        const int4 k4 = k4n;
+        double kd[7];
+#pragma unroll
+        for (int e = 0; e < 7; ++e) kd[e] = kdn[e];
+        if (q + 1 < q_end) {
+            k4n = a.ket.pint[q + 1];
+#pragma unroll
+            for (int e = 0; e < 7; ++e) kdn[e] = a.ket.pdbl[8 * (q + 1) + e];
+        }
         const int ksh = __builtin_amdgcn_readfirstlane(k4.x);
         const int lsh = __builtin_amdgcn_readfirstlane(k4.y);
         const int k0 = __builtin_amdgcn_readfirstlane(k4.z) - ao_base;
         const int l0 = __builtin_amdgcn_readfirstlane(k4.w) - ao_base;
-        const double *kd = a.ket.pdbl + 8 * q;
         const double qq = kd[0];
         const double Cc[3] = {kd[1], kd[2], kd[3]};
         const double CD[3] = {kd[4], kd[5], kd[6]};

@@ -326,6 +326,21 @@ PairSet pair_set(const mjk_basis_prod *bp, const ClassType &ct)
 
 int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
 
+// Classes (canonical orientation) whose FP32 tiles ran faster than their FP64 tiles on
+// MI355X ((H2O)64/def2-TZVP, per-tile split at a 1e-7 bound).  Without packed math, CDNA4
+// issues FP32 VALU work at the FP64 rate, so FP32 only pays where halved registers or LDS
+// raise occupancy; elsewhere the conversions cost more and the class stays FP64, which is
+// also the more accurate choice.
+bool fp32_pays(int la, int lb, int lc, int ld)
+{
+    static const int table[][4] = {{3, 2, 1, 1}, {3, 1, 2, 0}, {3, 2, 2, 2}, {3, 1, 1, 1}, {2, 2, 2, 0},
+                                   {3, 1, 1, 0}, {3, 0, 0, 0}, {3, 2, 2, 0}, {2, 2, 2, 2}, {2, 1, 1, 1},
+                                   {3, 2, 3, 0}};
+    for (const auto &c : table)
+        if (c[0] == la && c[1] == lb && c[2] == lc && c[3] == ld) return true;
+    return false;
+}
+
 int run_class_pair(mjk_basis_prod *bp, double *vj, double *vk, const double *dm, int n_dm,
                    int cp_ij, int cp_kl, const mjk_screen &sc, bool dens, hipStream_t st)
 {
@@ -346,7 +361,7 @@ int run_class_pair(mjk_basis_prod *bp, double *vj, double *vk, const double *dm,
     a.prim_tol = sc.direct_scf_tol * 1e-6;
     a.nbas = bp->nbas; a.nao = bp->nao; a.n_dm = n_dm;
     a.same = cp_ij == cp_kl;
-    const double pc = sc.precision_cutoff;
+    const double pc = fp32_pays(bra->l_bra, bra->l_ket, ket->l_bra, ket->l_ket) ? sc.precision_cutoff : 0.0;
     long nt = 0;
     // FP64 part: bound >= precision_cutoff (everything when the cutoff is 0)
     a.lo = pc > 0 ? pc : 0.0;
@@ -564,7 +579,7 @@ int batch_launch_cp(mjk_batch *b, const BatchCP &c, double *vj, double *vk, cons
         a.mols = b->d_mols;
         a.ndesc = c.ntiles;
     }
-    const double pc = sc.precision_cutoff;
+    const double pc = fp32_pays(c.l[0], c.l[1], c.l[2], c.l[3]) ? sc.precision_cutoff : 0.0;
     a.lo = pc > 0 ? pc : 0.0;
     a.hi = INFINITY;
     long nt = 0;

@@ -5,8 +5,8 @@
 
 Data: seeded water clusters (O-H 0.957 A, 104.5 deg; sizes uniform in [min, max]; box
 edge (n / 0.0156)^(1/3) A, at least 5 A, i.e. liquid-water density) converged by
-`byteqc_amd.scf.rhf.RHF` from SAD; every pair block within R_PAIR in its local frame is
-one sample.  Training: the perceptron kernels of libmi355jk.so (bf16 MFMA forward and
+`byteqc_amd.scf.rhf.RHF` from SAD; the target is D minus the superposition of fragment
+densities, and every pair block within R_PAIR of it, in its local frame, is one sample.  Training: the perceptron kernels of libmi355jk.so (bf16 MFMA forward and
 backward, Adam), minibatches drawn on the device, loss = mean squared block error in
 density units over the real (unpadded) entries.  Training seeds start at TRAIN_SEED0;
 held-out evaluation seeds at EVAL_SEED0, so the two sets never share a geometry.
@@ -21,6 +21,7 @@ import time
 import numpy as np
 
 from . import frames as fr
+from .fragments import fragment_density
 from .learned import Model, learned_guess
 
 TRAIN_SEED0 = 10_000
@@ -62,7 +63,7 @@ def make_dataset(basis, n_clusters, min_w, max_w, seed0=TRAIN_SEED0, conv_tol=1e
         if not mf.converged:
             log(f'[train] cluster {c} ({n} waters) did not converge; skipped')
             continue
-        dm = mf.dm.cpu().numpy()
+        dm = mf.dm.cpu().numpy() - fragment_density(mol)
         sym, xyz = fr.mol_geometry(mol)
         f, frames, types, pairs = fr.features(sym, xyz)
         X.append(f)
@@ -149,7 +150,7 @@ def fit(X, Y, types, clusters, masks, basis, elements, nmax, hidden=256, epochs=
     data = dict(mlp.params())
     data.update(dims=np.asarray(dims), nmax=np.asarray(nmax), basis=np.asarray(str(basis).lower()),
                 elements=np.asarray(elements), x_mean=x_mean.astype(np.float32), x_std=x_std.astype(np.float32),
-                y_mean=y_mean, y_std=y_std, loss_weights=w.astype(np.float32),
+                y_mean=y_mean, y_std=y_std, loss_weights=w.astype(np.float32), baseline=np.asarray('fragments'),
                 history=np.asarray(hist, dtype=np.float64),
                 train_info=np.asarray([len(X), ntr, epochs, batch, lr, time.perf_counter() - t0]))
     return Model(data)

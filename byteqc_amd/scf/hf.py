@@ -247,7 +247,7 @@ def _split_l_ctr_groups(uniq_l_ctr, l_ctr_counts, group_size):
 
 
 def jk_device(vhfopt, dm_cart, with_j=True, with_k=True, precision_cutoff=0.0,
-              density_screen=True, stream=None):
+              density_screen=True, stream=None, direct_scf_tol=None):
     """J/K in the Cartesian basis from a device density (n_dm, nao, nao) torch tensor.
 
     Returns device tensors (vj, vk) already symmetrised: J = 2 (V + V^T), K = V + V^T.
@@ -259,7 +259,8 @@ def jk_device(vhfopt, dm_cart, with_j=True, with_k=True, precision_cutoff=0.0,
     n_dm = dm_cart.shape[0]
     vj = torch.empty_like(dm_cart) if with_j else None
     vk = torch.empty_like(dm_cart) if with_k else None
-    sc = mjk.Screen(vhfopt.direct_scf_tol, float(precision_cutoff), int(bool(density_screen)))
+    tol = vhfopt.direct_scf_tol if direct_scf_tol is None else max(float(direct_scf_tol), vhfopt.direct_scf_tol)
+    sc = mjk.Screen(tol, float(precision_cutoff), int(bool(density_screen)))
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     mjk.check(lib.mjk_build_jk_all(vhfopt.bp, mjk.ptr(vj), mjk.ptr(vk), mjk.ptr(dm_cart), nao,
@@ -270,7 +271,7 @@ def jk_device(vhfopt, dm_cart, with_j=True, with_k=True, precision_cutoff=0.0,
 
 def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None, verbose=None,
            group_size=None, gpus=None, prescreen=True, nbins=None, *, precision_cutoff=0.0,
-           density_screen=None, out_device=False):
+           density_screen=None, out_device=False, direct_scf_tol=None):
     """Coulomb and exchange matrices for symmetric density matrices (hermi=1).
 
     dm: (nao, nao) or (n_dm, nao, nao) in the spherical (or mol.cart) AO basis, numpy or
@@ -279,6 +280,8 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None, 
     precision_cutoff > 0 runs quartets whose bound q_ij q_kl max|D| is below it in FP32.
     density_screen (default: prescreen) weights the Schwarz bound by max |D| over the six
     shell blocks a quartet touches; nbins is accepted for compatibility and unused.
+    direct_scf_tol (default: the plan's cutoff) screens quartets at a looser tolerance for
+    this call only; it cannot go below the cutoff the pair lists were built with.
     """
     if hermi != 1:
         raise NotImplementedError('JK-builder only supports hermitian density matrix')
@@ -299,7 +302,8 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None, 
     shape = dms.shape
     dms = dms.reshape(-1, nao0, nao0)
     dmc = torch.matmul(torch.matmul(c, dms), ct)
-    vj, vk = jk_device(vhfopt, dmc, with_j, with_k, precision_cutoff, density_screen)
+    vj, vk = jk_device(vhfopt, dmc, with_j, with_k, precision_cutoff, density_screen,
+                       direct_scf_tol=direct_scf_tol)
     out = []
     for v in (vj, vk):
         if v is None:

@@ -158,7 +158,7 @@ class RHF:
 
     def __init__(self, mol, conv_tol=1e-8, conv_tol_grad=None, max_cycle=100, diis_space=8,
                  direct_scf_tol=1e-13, precision_cutoff=0.0, incremental=True,
-                 rebuild_every=8, verbose=0, jk=None):
+                 rebuild_every=8, verbose=0, jk=None, early_tol=None, early_until=1e-3):
         self.mol = mol
         self.conv_tol = conv_tol
         self.conv_tol_grad = conv_tol_grad if conv_tol_grad is not None else np.sqrt(conv_tol) * 0.1
@@ -170,6 +170,12 @@ class RHF:
         self.rebuild_every = rebuild_every
         self.verbose = verbose
         self.jk = jk
+        # looser quartet screening while the orbital gradient max|[F, D]| exceeds early_until;
+        # the switch to direct_scf_tol forces a full rebuild, so the converged energy is
+        # computed at the tight tolerance
+        self.early_tol = early_tol
+        self.early_until = early_until
+        self._tol_now = None
         self._opt_gpu = None
         self.converged = False
         self.e_tot = None
@@ -198,6 +204,9 @@ class RHF:
     def get_init_guess(self, key='sad'):
         if key in ('sad', 'atom', 'minao'):
             return init_guess_by_atom(self.mol)
+        if key == 'fragments':
+            from ..guess.fragments import fragment_density
+            return fragment_density(self.mol)
         if key == 'learned':
             from ..guess.learned import learned_guess
             return learned_guess(self.mol, self._s)
@@ -215,7 +224,8 @@ class RHF:
             return self.jk(mol, dm, hermi=hermi, with_j=with_j, with_k=with_k)
         from . import hf
         return hf.get_jk(mol, dm, hermi, self._opt_gpu, with_j, with_k, omega,
-                         precision_cutoff=self.precision_cutoff, out_device=True)
+                         precision_cutoff=self.precision_cutoff, out_device=True,
+                         direct_scf_tol=self._tol_now)
 
     def make_rdm1(self, mo_coeff=None, mo_occ=None):
         torch = _torch()
@@ -250,8 +260,14 @@ class RHF:
         self.converged = False
         since_full = 0
         t_jk = 0.0
+        loose = self.early_tol is not None and self.early_tol > self.direct_scf_tol
+        self._tol_now = self.early_tol if loose else None
         for cycle in range(self.max_cycle):
             tj = time.perf_counter()
+            if loose and self.history and self.history[-1]['grad'] < self.early_until:
+                loose = False
+                self._tol_now = None
+                since_full = self.rebuild_every  # full rebuild at the tight tolerance
             if self.incremental and dm_last is not None and since_full < self.rebuild_every:
                 dj, dk = self.get_jk(mol, dm - dm_last)
                 vj = vj + dj
@@ -269,10 +285,11 @@ class RHF:
             gnorm = float(torch.abs(err).max())
             de = None if e_last is None else e_tot - e_last
             self.history.append({'cycle': cycle, 'e_tot': e_tot, 'delta_e': de, 'grad': gnorm,
-                                 't': time.perf_counter() - t_start, 't_jk': dt_jk})
+                                 't': time.perf_counter() - t_start, 't_jk': dt_jk,
+                                 'tol': self._tol_now or self.direct_scf_tol})
             if self.verbose:
                 print(f'cycle {cycle}: E = {e_tot:.12f}  dE = {de}  |[F,D]| = {gnorm:.2e}', flush=True)
-            if de is not None and abs(de) < self.conv_tol and gnorm < self.conv_tol_grad:
+            if de is not None and abs(de) < self.conv_tol and gnorm < self.conv_tol_grad and not loose:
                 self.converged = True
                 e_last = e_tot
                 break

@@ -157,3 +157,24 @@ def test_learned_guess_beats_sad_on_held_out_clusters():
         assert r['learned']['cycles'] < r['sad']['cycles']
         e = r['sad']['e_tot']
         assert abs(r['learned']['e_first'] - e) < abs(r['sad']['e_first'] - e)
+
+
+def test_fragment_baseline_is_exact_for_distant_copies():
+    """Two rigid waters 20 A apart in different orientations: the converged density equals the
+    superposition of the isolated water's density rotated into each orientation (what remains
+    is their interaction, far below the 1e-4 tolerance; a wrong rotation convention gives
+    errors of order 0.1)."""
+    from byteqc_amd.gto.clusters import _rotation, water_monomer
+    from byteqc_amd.gto.mole import Mole
+    from byteqc_amd.guess.fragments import fragment_density
+    from byteqc_amd.scf.rhf import RHF
+    rng = np.random.default_rng(12)
+    mono = water_monomer()
+    xyz = np.concatenate([mono @ _rotation(rng.normal(size=4)).T,
+                          mono @ _rotation(rng.normal(size=4)).T + np.array([20.0, 0.0, 0.0])])
+    mol = Mole([(('O', 'H', 'H')[i % 3], tuple(p)) for i, p in enumerate(xyz)], basis='cc-pvdz')
+    mf = RHF(mol, conv_tol=1e-10)
+    mf.kernel()
+    assert mf.converged
+    d = mf.dm.cpu().numpy()
+    assert np.abs(fragment_density(mol) - d).max() < 1e-4

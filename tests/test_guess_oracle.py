@@ -104,3 +104,26 @@ def test_features_invariant_blocks_covariant(basis):
     Dr = fr.assemble(mol, b1, pairs, F, nmax)
     covered = Dr != 0
     assert np.abs((Dr - D)[covered]).max() < 1e-5 * np.abs(D).max()
+
+
+def test_fragments_and_alignment():
+    """Water clusters split into their molecules; every molecule aligns onto the first with
+    rmsd below 1e-3 A, including a copy whose hydrogens are listed in the other order."""
+    from byteqc_amd.guess import fragments as fg
+    mol = water_cluster(6, 8.0, 21, basis='cc-pvdz')
+    sym, xyz = fr.mol_geometry(mol)
+    frags = fg.find_fragments(sym, xyz)
+    assert sorted(len(f) for f in frags) == [3] * 6
+    ref = frags[0]
+    for f in frags[1:]:
+        r, perm = fg.align([sym[a] for a in ref], xyz[ref], [sym[a] for a in f], xyz[f])
+        assert np.allclose(r @ r.T, np.eye(3)) and abs(np.linalg.det(r) - 1) < 1e-12
+        moved = (xyz[ref] - xyz[ref].mean(0)) @ r.T + xyz[f].mean(0)
+        assert np.abs(moved - xyz[f][perm]).max() < 1e-3
+    swapped = [f[0], f[2], f[1]]
+    r, perm = fg.align([sym[a] for a in ref], xyz[ref], [sym[a] for a in swapped], xyz[swapped])
+    assert np.abs((xyz[ref] - xyz[ref].mean(0)) @ r.T + xyz[swapped].mean(0) - xyz[swapped][perm]).max() < 1e-3
+    # a distorted copy is not a rigid copy
+    bent = xyz[f].copy()
+    bent[1] += 0.05
+    assert fg.align([sym[a] for a in ref], xyz[ref], [sym[a] for a in f], bent) is None
