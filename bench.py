@@ -113,6 +113,17 @@ def run_batch(n_mol, reps, torch):
     return out
 
 
+def measured_traffic(workload, symbol):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy) of a kernel
+    symbol, from the rocprofv3 --pmc runs recorded for this workload; None if not recorded."""
+    path = os.path.join(ROOT, 'byteqc_amd', 'perf', f'traffic_{workload.replace("-", "_")}.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)['kernels'].get(symbol)
+    return None if rec is None else rec['fetch_bytes_per_launch'] + rec['write_bytes_per_launch']
+
+
 def per_class_profile(vhfopt, dmc, vj, vk, screen, torch, lib, mjk):
     """Time every class pair separately (HIP events on the launch stream)."""
     from byteqc_amd.perf.flops import class_flops, class_label
@@ -303,15 +314,18 @@ def main(argv=None):
         dname, dom = max(sym.items(), key=lambda kv: kv[1]['ms'])
         ach = dom['flops'] / (dom['ms'] * 1e-3) / 1e12
         from byteqc_amd.perf.flops import kernel_symbol
+        sym_name = kernel_symbol(dname)
         result['roofline'] = {
             'bound': 'mfma', 'achieved': ach, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-            'frac': ach / FP64_PEAK_TFLOPS, 'traffic': None,
-            'kernel': kernel_symbol(dname),
+            'frac': ach / FP64_PEAK_TFLOPS, 'traffic': measured_traffic(args.workload, sym_name),
+            'kernel': sym_name,
             'launches': dom['launches'], 'avg_launch_ms': dom['ms'] / dom['launches'],
             'quartets': dom['quartets'], 'flops_per_build': dom['flops'],
             'note': 'FP64 vector-ALU roof (Rys recursion + digestion); MI355X FP64 vector and '
                     'matrix peaks are both 78.6 TFLOP/s.  achieved = algorithmic FLOPs of all '
-                    'launches of this kernel symbol in one build / their summed HIP-event time',
+                    'launches of this kernel symbol in one build / their summed HIP-event time; '
+                    'traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE '
+                    '(calibrated on a 1 GiB copy: x2.0 and x1.0), mostly FP64 atomics to J/K',
         }
         result['build_tflops'] = tot_fl / (tot_ms * 1e-3) / 1e12
         if args.classes_out:
