@@ -811,6 +811,39 @@ int mjk_batch_time_class_pair(mjk_batch *b, int i, double *vj, double *vk, const
     return rc;
 }
 
+int mjk_fill_int2e(mjk_basis_prod *bp, double *eri, int nao, const int64_t *strides, const int32_t *ao_offsets,
+                   int cp_ij, int cp_kl, void *stream)
+{
+    if (!bp || !eri || !strides || !ao_offsets) return fail(MJK_ERR_ARG, "mjk_fill_int2e: null argument");
+    if (nao != bp->nao) return fail(MJK_ERR_ARG, "mjk_fill_int2e: nao does not match the cache");
+    if (cp_ij < 0 || cp_kl < 0 || cp_ij >= bp->ncptype || cp_kl >= bp->ncptype)
+        return fail(MJK_ERR_ARG, "mjk_fill_int2e: class index out of range");
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != bp->device) return fail(MJK_ERR_ARG, "mjk_fill_int2e: current device differs from the cache's device");
+    const ClassType &c1 = bp->cp[cp_ij], &c2 = bp->cp[cp_kl];
+    if (c1.npairs == 0 || c2.npairs == 0) return MJK_OK;
+    // the kernels take the class with more Cartesian products as the bra; the strides follow
+    const bool swap = ncart_h(c1.l_bra) * ncart_h(c1.l_ket) < ncart_h(c2.l_bra) * ncart_h(c2.l_ket);
+    const ClassType &b = swap ? c2 : c1, &k = swap ? c1 : c2;
+    FillArgs f{};
+    f.bra = pair_set(bp, b);
+    f.ket = pair_set(bp, k);
+    f.eri = eri;
+    const int ib = swap ? 2 : 0, ik = swap ? 0 : 2;
+    f.sb0 = (long)strides[ib];
+    f.sb1 = (long)strides[ib + 1];
+    f.sk0 = (long)strides[ik];
+    f.sk1 = (long)strides[ik + 1];
+    f.ob0 = ao_offsets[ib];
+    f.ob1 = ao_offsets[ib + 1];
+    f.ok0 = ao_offsets[ik];
+    f.ok1 = ao_offsets[ik + 1];
+    hipError_t e = launch_fill_class(b.l_bra, b.l_ket, k.l_bra, k.l_ket, f, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "mjk_fill_int2e: launch");
+    return MJK_OK;
+}
+
 int mjk_batch_stats(mjk_batch *b, double *out)
 {
     if (!b) return fail(MJK_ERR_ARG, "mjk_batch_stats: null batch");

@@ -30,6 +30,16 @@ hipError_t launch_symmetrize_batch(double *v, const MolRec *mols, int n_mol, int
 // the bra pairs one of its tiles covers.
 int class_kernel_info(int la, int lb, int lc, int ld, int *bra_per_tile);
 
+// One class pair of an ERI-tensor fill (fill_int2e.hip).
+struct FillArgs {
+    PairSet bra, ket;
+    double *eri;
+    long sb0, sb1, sk0, sk1;  // element strides of the bra's i, j and the ket's k, l
+    int ob0, ob1, ok0, ok1;   // AO offsets subtracted from them
+    long base;                // first quartet of this launch
+};
+hipError_t launch_fill_class(int la, int lb, int lc, int ld, FillArgs f, hipStream_t st);
+
 hipError_t launch_class(int la, int lb, int lc, int ld, bool fp32, const JKArgs &a, hipStream_t st, long *nt);
 
 }  // namespace mjk

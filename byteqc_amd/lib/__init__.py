@@ -33,7 +33,7 @@ EXPORTS = ('mjk_version', 'mjk_last_error', 'mjk_q_cond', 'mjk_init_basis_prod',
            'mjk_last_stats', 'mjk_int1e', 'mjk_gemm_bf16', 'mjk_bias_act', 'mjk_act_backward',
            'mjk_mse_grad', 'mjk_adam', 'mjk_to_bf16', 'mjk_batch_create', 'mjk_batch_destroy',
            'mjk_batch_build', 'mjk_batch_num_class_pairs', 'mjk_batch_tiles', 'mjk_batch_set_kc',
-           'mjk_batch_time_class_pair', 'mjk_batch_stats')
+           'mjk_batch_time_class_pair', 'mjk_batch_stats', 'mjk_fill_int2e')
 
 _lib = None
 
@@ -79,6 +79,7 @@ def load():
     lib.mjk_batch_time_class_pair.argtypes = [p, i, p, p, p, ctypes.POINTER(Screen), i, p,
                                               ctypes.POINTER(ctypes.c_float)]
     lib.mjk_batch_stats.argtypes = [p, p]
+    lib.mjk_fill_int2e.argtypes = [p, p, i, p, p, i, i, p]
     _lib = lib
     return lib
 

@@ -137,6 +137,7 @@ class _VHFOpt:
         self.uniq_l_ctr = uniq
         self.l_ctr_offsets = np.append(0, np.cumsum(counts))
         self.direct_scf_tol = cutoff
+        self.diag_block_with_triu = bool(diag_block_with_triu)
 
         lib = mjk.load()
         q = np.zeros((self.nbas, self.nbas))

@@ -109,6 +109,17 @@ int mjk_last_stats(mjk_basis_prod *bp, double *out);
 int mjk_int1e(const int32_t *atm, int natm, const int32_t *bas, int nbas, const double *env,
               double *s, double *t, double *v, int nthreads);
 
+/* Add the Cartesian ERI block (ij|kl) of every pair of bra class cp_ij and ket class cp_kl
+ * into eri (device), element (a, b, c, d) of shells (i, j, k, l) at
+ *   sum over (ao_i + a - ao_offsets[0]) * strides[0], (ao_j + b - ao_offsets[1]) * strides[1],
+ *            (ao_k + c - ao_offsets[2]) * strides[2], (ao_l + d - ao_offsets[3]) * strides[3].
+ * Replaces GINTfill_int2e (cuobc/lib/gint/nr_fill_ao_ints.cu:493-551) without its bins: every
+ * listed pair of both classes is computed.  Build the cache with diag_fac = 1 (full pair
+ * lists on diagonal groups, as get_int4c's diag_block_with_triu) for plain ERIs.  Strides are
+ * 64-bit (the reference's are int). */
+int mjk_fill_int2e(mjk_basis_prod *bp, double *eri, int nao, const int64_t *strides,
+                   const int32_t *ao_offsets, int cp_ij, int cp_kl, void *stream);
+
 /* ---- Batched builds: many molecules, one launch per class pair ------------------------
  * No counterpart in the reference, whose get_jk handles one molecule per call and whose
  * multi-GPU map hands whole class pairs to devices (cuobc/lib/multigpu.py:264-291).
