@@ -291,6 +291,15 @@ __device__ inline void rys2d(R g0, R c00, R c0p, R b00, R b10, R b01, R ab, R cd
 }
 
 // ---- per-class launch configuration ----------------------------------------------
+// Classes (canonical orientation) under the register limit that ran faster on the
+// wave-group kernel, measured per class on (H2O)64/def2-TZVP: (pp|ds) 93 -> 56 ms,
+// (ds|ds) 47 -> 35, (dd|ss) 31 -> 19 ((dp|ps) went 77 -> 93 and stays on the register kernel).
+__host__ __device__ constexpr bool prefer_group(int la, int lb, int lc, int ld)
+{
+    return (la == 1 && lb == 1 && lc == 2 && ld == 0) || (la == 2 && lb == 0 && lc == 2 && ld == 0) ||
+           (la == 2 && lb == 2 && lc == 0 && ld == 0);
+}
+
 #ifndef REG_LIMIT
 #define REG_LIMIT 150
 #endif
@@ -303,7 +312,8 @@ struct Cfg {
     static constexpr int GS = (LA + 1) * (LB + 1) * G2;
     // register-resident classes: one root's 2D tables, the quartet's integrals and the
     // per-lane exchange/Coulomb partial sums fit the 512-entry VGPR+AGPR file
-    static constexpr bool REG = 3 * GS + NFAB * NFCD + NFAB + (NFA + NFB) * NFC <= REG_LIMIT;
+    static constexpr bool REG = 3 * GS + NFAB * NFCD + NFAB + (NFA + NFB) * NFC <= REG_LIMIT &&
+                                !prefer_group(LA, LB, LC, LD);
     static constexpr int ACC_MAX = 64;
     static constexpr int RPT0 = REG ? NFAB : cmax(1, cmin(NFAB, ACC_MAX / NFCD));
     static constexpr int S = REG ? 1 : (NFAB + RPT0 - 1) / RPT0;

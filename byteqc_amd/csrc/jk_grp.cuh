@@ -60,13 +60,14 @@ struct GCfg {
 
 // Classes whose group kernel runs faster compiled for two waves per SIMD (a few scratch
 // spills against twice the latency hiding), measured per class on (H2O)64/def2-TZVP:
-// (fp|fp) 107 -> 67 ms, (fd|fp) 53 -> 40, (fp|ds) 59 -> 47; where the spills are larger it
+// (fp|fp) 107 -> 67 ms, (fd|fp) 53 -> 40, (fp|ds) 59 -> 47, (dd|ss) 22 -> 19; where the spills are larger it
 // loses ((fp|dd) 64 -> 133) and the class keeps the compiler's default.
 constexpr int grp_waves_per_eu(int la, int lb, int lc, int ld)
 {
     constexpr int t[][4] = {{3, 1, 3, 1}, {3, 2, 3, 1}, {3, 1, 2, 0}, {3, 1, 1, 0}, {3, 2, 3, 0},
                             {3, 1, 1, 1}, {2, 2, 2, 0}, {3, 3, 3, 0}, {3, 2, 2, 0}, {3, 2, 2, 2},
-                            {3, 3, 2, 0}, {2, 2, 2, 1}, {3, 2, 1, 1}};
+                            {3, 3, 2, 0}, {2, 2, 2, 1}, {3, 2, 1, 1}, {2, 2, 0, 0}, {3, 3, 1, 1},
+                            {3, 2, 2, 1}};
     for (const auto &c : t)
         if (c[0] == la && c[1] == lb && c[2] == lc && c[3] == ld) return 2;
     return 1;

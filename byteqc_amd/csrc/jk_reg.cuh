@@ -22,7 +22,7 @@ namespace mjk {
 constexpr int REG_BRA = 256;  // bra pairs (lanes) per tile
 // Workgroups per CU each class is compiled for.  Two halve the registers a lane may hold:
 // faster where the extra wave hides more latency than the spills cost, measured per class
-// on (H2O)64/def2-TZVP ((fp|ss) 92 -> 70 ms, (ds|pp) 95 -> 86, (dp|ss) 68 -> 60, while
+// on (H2O)64/def2-TZVP ((fp|ss) 92 -> 70 ms, (dp|ss) 68 -> 60, while
 // (dp|ps) went 80 -> 112 and stays at one).
 constexpr int reg_min_blocks(int la, int lb, int lc, int ld)
 {
@@ -30,7 +30,6 @@ constexpr int reg_min_blocks(int la, int lb, int lc, int ld)
     switch (key) {
     case ((3 * 4 + 1) * 4 + 0) * 4 + 0:  // (fp|ss)
     case ((3 * 4 + 0) * 4 + 1) * 4 + 0:  // (fs|ps)
-    case ((2 * 4 + 0) * 4 + 1) * 4 + 1:  // (ds|pp)
     case ((2 * 4 + 1) * 4 + 0) * 4 + 0:  // (dp|ss)
     case ((2 * 4 + 0) * 4 + 1) * 4 + 0:  // (ds|ps)
     case ((1 * 4 + 1) * 4 + 1) * 4 + 0:  // (pp|ps)

@@ -48,9 +48,12 @@ def kernel_symbol(label, fp32=False):
     import re
     la, lb, lc, ld = ['spdfg'.index(x) for x in re.findall('[spdfg]', label)]
     nc = ncart
+    if nc(la) * nc(lb) < nc(lc) * nc(ld):  # the kernels take the pair with more products as bra
+        la, lb, lc, ld = lc, ld, la, lb
     gs = (la + 1) * (lb + 1) * (lc + 1) * (ld + 1)
     regs = 3 * gs + nc(la) * nc(lb) * nc(lc) * nc(ld) + nc(la) * nc(lb) + (nc(la) + nc(lb)) * nc(lc)
-    name = 'jk_reg_kernel' if regs <= 150 else 'jk_grp_kernel'
+    group = (la, lb, lc, ld) in ((1, 1, 2, 0), (2, 0, 2, 0), (2, 2, 0, 0))  # measured faster on the group kernel
+    name = 'jk_reg_kernel' if regs <= 150 and not group else 'jk_grp_kernel'
     if (la, lb, lc, ld) in ((3, 3, 3, 2), (3, 3, 3, 3)):
         name = 'jk_kernel'  # the block-LDS kernel keeps the two largest classes
     return f'mjk::{name}<{"float" if fp32 else "double"}, {la}, {lb}, {lc}, {ld}>'
