@@ -69,8 +69,24 @@ class Model:
         return syms <= set(self.elements) and str(mol.basis).lower() == self.basis
 
 
-def learned_guess(mol, s, model=None):
-    """Initial density (nao, nao) from the trained model for mol's basis."""
+def learned_guess(mol, s, model=None, via=None):
+    """Initial density (nao, nao) from the trained model for mol's basis, or, with via=<basis>,
+    from the model of that basis projected into mol's basis (project.py)."""
+    if via is not None and str(via).lower() != str(mol.basis).lower():
+        from ..gto.mole import Mole
+        from .project import project_density
+        small = Mole(mol.atom, basis=via)
+        from ..scf.hf import basis_seg_contraction
+        from ..scf.rhf import int1e
+        from types import SimpleNamespace
+        bas, env, coeff = basis_seg_contraction(small)
+        s_small, _, _ = int1e(SimpleNamespace(atm=np.ascontiguousarray(small._atm, dtype=np.int32), bas=bas,
+                                              env=env, coeff=coeff))
+        dm = project_density(small, learned_guess(small, s_small, model), mol)
+        ne = float(np.sum(dm * s))
+        if ne > 0:
+            dm *= mol.nelectron / ne
+        return dm
     if model is None:
         path = model_path(mol.basis)
         if not os.path.exists(path):

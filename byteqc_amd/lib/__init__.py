@@ -50,6 +50,13 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f'{LIB_PATH} is missing: build it with `python -c "import '
                           f'__graft_entry__ as g; g.build()"` (hipcc, gfx950)')
+    try:
+        # let torch create its HIP context before the library makes its first HIP call: the
+        # Python host hands torch device memory and streams to the library
+        import torch
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     p, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     lib.mjk_version.restype = i

@@ -210,6 +210,10 @@ class RHF:
         if key == 'learned':
             from ..guess.learned import learned_guess
             return learned_guess(self.mol, self._s)
+        if key.startswith('learned:'):
+            # the model of another basis (e.g. 'learned:cc-pvdz'), projected into this one
+            from ..guess.learned import learned_guess
+            return learned_guess(self.mol, self._s, via=key.split(':', 1)[1])
         if key in ('1e', 'hcore'):
             torch = _torch()
             e, c = eig(torch.as_tensor(self._h, device='cuda'), torch.as_tensor(self._s, device='cuda'))

@@ -127,3 +127,35 @@ def test_fragments_and_alignment():
     bent = xyz[f].copy()
     bent[1] += 0.05
     assert fg.align([sym[a] for a in ref], xyz[ref], [sym[a] for a in f], bent) is None
+
+
+def test_projection_between_bases():
+    """Self-projection is the identity; the 'to' block of the combined overlap equals the
+    'to' basis' own overlap; projecting the cc-pVDZ overlap inverse (tr(D S) = AO count) into
+    def2-TZVP loses weight but little."""
+    from byteqc_amd.guess.project import cross_overlap, project_density
+    from byteqc_amd.gto.mole import Mole
+    from byteqc_amd.scf.hf import basis_seg_contraction
+    from byteqc_amd.scf.rhf import int1e
+    from types import SimpleNamespace
+
+    def overlap(m):
+        bas, env, coeff = basis_seg_contraction(m)
+        return int1e(SimpleNamespace(atm=np.ascontiguousarray(m._atm, dtype=np.int32), bas=bas, env=env,
+                                     coeff=coeff))[0]
+
+    mol = water_cluster(2, 6.0, 3, basis='cc-pvdz')
+    tz = Mole(mol.atom, basis='def2-tzvp')
+    rng = np.random.default_rng(9)
+    d = rng.normal(size=(mol.nao, mol.nao))
+    d = d + d.T
+    assert np.allclose(project_density(mol, d, mol), d, atol=1e-9)
+    s_tt, s_tf = cross_overlap(mol, tz)
+    assert np.allclose(s_tt, overlap(tz), atol=1e-12)
+    s_ff = overlap(mol)
+    dz = np.linalg.inv(s_ff)            # tr(dz s_ff) = nao_dz
+    dt = project_density(mol, dz, tz)
+    # a projection can only lose weight: what def2-TZVP cannot represent of arbitrary
+    # cc-pVDZ functions (about 3% here)
+    tr = np.sum(dt * s_tt)
+    assert 0.9 * mol.nao < tr <= mol.nao + 1e-9
