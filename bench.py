@@ -17,8 +17,8 @@ Also reported:
                 screening and density, timed on a systematic sample of bra-pair rows and
                 scaled to the full build
   scf           (--scf) RHF time-to-1e-8 Eh: FP64 + SAD guess (the reference's path) against
-                FP64 + learned guess, mixed (FP32 quartets below a 1e-7 bound, 1e-10
-                screening until max|[F,D]| < 1e-3) + SAD, and mixed + learned
+                FP64 + learned guess, mixed (FP32 quartets below a 1e-7 bound, 1e-9
+                screening until max|[F,D]| < 1e-4) + SAD, and mixed + learned
   batch         (--batch N) N seeded water clusters of 1-32 molecules, 6-31G*, random PSD
                 densities (D = 2 Q Q^T, trace = electron count): all N J/K builds as a
                 per-molecule loop of mjk_build_jk_all, as one batched build, as a replayed
@@ -196,19 +196,19 @@ def cpu_baseline(vhfopt, dm_cart_host, target_s):
                       f'shell pairs each, {t_all:.1f} s measured), scaled by {stride}'}
 
 
-def run_scf(mol, precision_cutoff=0.0, guess='sad', early_tol=None):
+def run_scf(mol, precision_cutoff=0.0, guess='sad', early_tol=None, early_until=1e-3):
     """RHF to |dE| < 1e-8 Eh and max|[F,D]| < 1e-5; wall time from the guess to convergence
     (guess included; the SAD atomic densities are cached beforehand, as for a warm code)."""
     from byteqc_amd.scf.rhf import RHF, init_guess_by_atom
     init_guess_by_atom(mol)  # warm the atomic-density cache
-    mf = RHF(mol, conv_tol=1e-8, precision_cutoff=precision_cutoff, early_tol=early_tol)
+    mf = RHF(mol, conv_tol=1e-8, precision_cutoff=precision_cutoff, early_tol=early_tol, early_until=early_until)
     mf.build()
     e = mf.kernel(init_guess=guess)
     return {'e_tot': e, 'converged': mf.converged, 'cycles': mf.cycles, 'guess': guess,
             'wall_s': mf.timings['guess'] + mf.timings['scf'],
             'time_s': mf.timings['scf'], 'guess_s': mf.timings['guess'], 'jk_s': mf.timings['jk'],
             'setup_s': mf.timings['vhfopt'] + mf.timings['int1e'],
-            'precision_cutoff': precision_cutoff, 'early_tol': early_tol,
+            'precision_cutoff': precision_cutoff, 'early_tol': early_tol, 'early_until': early_until,
             'cycle_jk_s': [round(h['t_jk'], 4) for h in mf.history],
             'cycle_e': [h['e_tot'] for h in mf.history],
             'cycle_grad': [h['grad'] for h in mf.history]}
@@ -348,8 +348,8 @@ def main(argv=None):
         learned = 'learned' if os.path.exists(model_path(mol.basis)) else 'sad'
         runs = {'fp64_sad': dict(),
                 'fp64_learned': dict(guess=learned),
-                'mixed_sad': dict(precision_cutoff=1e-7, early_tol=1e-10),
-                'mixed_learned': dict(precision_cutoff=1e-7, early_tol=1e-10, guess=learned)}
+                'mixed_sad': dict(precision_cutoff=1e-7, early_tol=1e-9, early_until=1e-4),
+                'mixed_learned': dict(precision_cutoff=1e-7, early_tol=1e-9, early_until=1e-4, guess=learned)}
         wanted = set(args.scf_runs.split(',')) | {'fp64_sad'}
         runs = {k: v for k, v in runs.items() if k in wanted}
         r = result['scf'] = {}
