@@ -402,10 +402,10 @@ jk_reg_kernel(const JKArgs a)
 #pragma unroll
                     for (int d = 0; d < NFD; ++d) atomicAdd(&kjl_slot[wv][jlead][x * NFD + d], kjl[x][d]);
             }
-            // the wave's LDS adds complete before its leaders read the slots
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            // LDS instructions of one wave execute in issue order, so the leaders' reads below
+            // see every lane's add; only the compiler must not reorder them (a memory fence here
+            // would also wait for the wave's outstanding global atomics every ket)
             __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (jlead == lane && pvalid) {
 #pragma unroll
                 for (int x = 0; x < NFB; ++x)
