@@ -12,7 +12,6 @@ real PySCF ``Mole`` can be handed to the same driver.
 from __future__ import annotations
 
 import json
-import math
 import os
 from functools import lru_cache
 

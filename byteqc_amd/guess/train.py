@@ -37,14 +37,16 @@ def _log(msg):
     print(msg, flush=True)
 
 
-def converged(mol, conv_tol=1e-8, init_guess='sad', dm0=None, precision_cutoff=0.0):
+def converged(mol, conv_tol=1e-8, init_guess='sad', dm0=None, precision_cutoff=0.0, early_tol=None):
     from ..scf.rhf import RHF
-    mf = RHF(mol, conv_tol=conv_tol, precision_cutoff=precision_cutoff)
+    mf = RHF(mol, conv_tol=conv_tol, precision_cutoff=precision_cutoff, early_tol=early_tol,
+             early_until=1e-4)
     mf.kernel(dm0=dm0, init_guess=init_guess)
     return mf
 
 
-def make_dataset(basis, n_clusters, min_w, max_w, seed0=TRAIN_SEED0, conv_tol=1e-7, max_seconds=None, log=_log):
+def make_dataset(basis, n_clusters, min_w, max_w, seed0=TRAIN_SEED0, conv_tol=1e-7, max_seconds=None, early_tol=None,
+                 log=_log):
     from ..gto.clusters import water_cluster
     rng = np.random.default_rng(seed0)
     sizes = rng.integers(min_w, max_w + 1, n_clusters)
@@ -59,7 +61,7 @@ def make_dataset(basis, n_clusters, min_w, max_w, seed0=TRAIN_SEED0, conv_tol=1e
         mol = water_cluster(n, box_for(n), seed0 + 1 + c, basis=basis)
         if nmax is None:
             nmax = max(v[1] for v in fr.element_layout(mol).values())
-        mf = converged(mol, conv_tol)
+        mf = converged(mol, conv_tol, early_tol=early_tol)
         if not mf.converged:
             log(f'[train] cluster {c} ({n} waters) did not converge; skipped')
             continue
@@ -191,12 +193,14 @@ def main(argv=None):
     ap.add_argument('--hidden', type=int, default=256)
     ap.add_argument('--conv-tol', type=float, default=1e-7)
     ap.add_argument('--max-seconds', type=float, default=None, help='stop generating data after this long')
+    ap.add_argument('--early-tol', type=float, default=None,
+                    help='looser quartet screening in the data SCFs until max|[F,D]| < 1e-4')
     ap.add_argument('--out', required=True)
     ap.add_argument('--eval-waters', type=int, default=32)
     ap.add_argument('--eval-seeds', type=int, default=2)
     a = ap.parse_args(argv)
     X, Y, T, C, nmax, mol = make_dataset(a.basis, a.clusters, a.min_waters, a.max_waters, conv_tol=a.conv_tol,
-                                         max_seconds=a.max_seconds)
+                                         max_seconds=a.max_seconds, early_tol=a.early_tol)
     _log(f'[train] {len(X)} pair samples, nmax {nmax}')
     masks = fr.block_masks(mol, nmax)
     elements = sorted({mol.atom_symbol(i) for i in range(mol.natm)})

@@ -19,7 +19,7 @@ import time
 import numpy as np
 
 from .. import lib as mjk
-from ..gto.mole import ANG_OF, NPRIM_OF, PTR_COEFF, PTR_EXP, ATOM_OF
+from ..gto.mole import ATOM_OF, PTR_COEFF, PTR_EXP
 from . import hf
 
 KC_CANDIDATES = {0: (16, 32, 64, 128, 256), 1: (4, 8, 16, 32, 64)}
@@ -58,7 +58,6 @@ class BatchJK:
     def _merge(self):
         opts = self.opts
         groups = sorted({tuple(int(v) for v in g) for o in opts for g in o.uniq_l_ctr})
-        gidx = {g: i for i, g in enumerate(groups)}
         ng = len(groups)
         atm, bas, env = [], [], []
         shell_off, ao_off = [], []

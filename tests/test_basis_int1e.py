@@ -1,5 +1,4 @@
 """Basis data, AO conventions and the library's one-electron integrals (CPU)."""
-import itertools
 
 import numpy as np
 import pytest

@@ -1,5 +1,4 @@
 """Mixed FP32/FP64 quartets and density-weighted screening against the FP64 build."""
-import numpy as np
 import pytest
 
 from byteqc_amd.gto import water_cluster
