@@ -215,7 +215,29 @@ def run_scf(mol, precision_cutoff=0.0, guess='sad', early_tol=None, early_until=
             'precision_cutoff': precision_cutoff, 'early_tol': early_tol, 'early_until': early_until,
             'cycle_jk_s': [round(h['t_jk'], 4) for h in mf.history],
             'cycle_e': [h['e_tot'] for h in mf.history],
-            'cycle_grad': [h['grad'] for h in mf.history]}
+            'cycle_grad': [h['grad'] for h in mf.history],
+            'device_mem_used_gb': device_mem_used_gb()}
+
+
+def device_mem_used_gb():
+    """Device memory in use (all allocators: torch's cache and the library's own buffers)."""
+    import torch
+    free, total = torch.cuda.mem_get_info()
+    return (total - free) / 1e9
+
+
+def run_scf_diagnosed(mol):
+    """The mixed + learned run again with plain FP64 J/K of every cycle's density built beside it
+    (outside the timings): max|dJ|, max|dK| and the energy difference per iteration."""
+    from byteqc_amd.guess.learned import model_path
+    from byteqc_amd.scf.rhf import RHF
+    guess = 'learned' if os.path.exists(model_path(mol.basis)) else 'sad'
+    mf = RHF(mol, conv_tol=1e-8, precision_cutoff=1e-7, early_tol=1e-9, early_until=1e-4, diagnose_jk=True)
+    mf.build()
+    mf.kernel(init_guess=guess)
+    return {'guess': guess, 'cycles': mf.cycles, 'e_tot': mf.e_tot,
+            'per_cycle': [{k: h[k] for k in ('cycle', 'tol', 'max_dj', 'max_dk', 'de_vs_fp64_fock', 'grad')}
+                          for h in mf.history]}
 
 
 def main(argv=None):
@@ -230,6 +252,8 @@ def main(argv=None):
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-classes', action='store_true')
     ap.add_argument('--scf', action='store_true')
+    ap.add_argument('--scf-diagnose', action='store_true',
+                    help='also rerun mixed + learned with per-cycle |dJ|, |dK|, dE against FP64 J/K')
     ap.add_argument('--scf-runs', default='fp64_sad,fp64_learned,mixed_sad,mixed_learned',
                     help='comma-separated subset of the SCF comparison (fp64_sad always runs)')
     ap.add_argument('--classes-out', default=None, help='write every class timing to this JSON file')
@@ -368,6 +392,12 @@ def main(argv=None):
         if 'mixed_learned' in r:
             log(f'scf mixed+learned vs fp64+SAD: {r["mixed_learned"]["speedup_vs_fp64_sad"]:.2f}x, '
                 f'dE = {r["mixed_learned"]["delta_e_vs_fp64_sad"]:.1e}')
+        if args.scf_diagnose:
+            r['mixed_learned_per_cycle'] = run_scf_diagnosed(mol)
+            worst = max(r['mixed_learned_per_cycle']['per_cycle'], key=lambda c: c['max_dk'])
+            log(f'scf mixed+learned per cycle vs FP64 J/K: worst max|dJ| '
+                f'{max(c["max_dj"] for c in r["mixed_learned_per_cycle"]["per_cycle"]):.1e}, '
+                f'max|dK| {worst["max_dk"]:.1e}')
     if args.batch:
         vhfopt.clear()
         torch.cuda.empty_cache()

@@ -158,7 +158,8 @@ class RHF:
 
     def __init__(self, mol, conv_tol=1e-8, conv_tol_grad=None, max_cycle=100, diis_space=8,
                  direct_scf_tol=1e-13, precision_cutoff=0.0, incremental=True,
-                 rebuild_every=8, verbose=0, jk=None, early_tol=None, early_until=1e-3):
+                 rebuild_every=8, verbose=0, jk=None, early_tol=None, early_until=1e-3,
+                 diagnose_jk=False):
         self.mol = mol
         self.conv_tol = conv_tol
         self.conv_tol_grad = conv_tol_grad if conv_tol_grad is not None else np.sqrt(conv_tol) * 0.1
@@ -176,6 +177,9 @@ class RHF:
         self.early_tol = early_tol
         self.early_until = early_until
         self._tol_now = None
+        # diagnose_jk: also build plain FP64 J/K (full, tight screening) of every cycle's density,
+        # outside the timings, and record max|dJ|, max|dK| and the energy it implies
+        self.diagnose_jk = diagnose_jk
         self._opt_gpu = None
         self.converged = False
         self.e_tot = None
@@ -264,6 +268,7 @@ class RHF:
         self.converged = False
         since_full = 0
         t_jk = 0.0
+        t_diag = 0.0
         loose = self.early_tol is not None and self.early_tol > self.direct_scf_tol
         self._tol_now = self.early_tol if loose else None
         for cycle in range(self.max_cycle):
@@ -285,12 +290,21 @@ class RHF:
             t_jk += dt_jk
             f = h + vj - 0.5 * vk
             e_tot = float(0.5 * torch.sum(dm * (h + f))) + enuc
+            diag = {}
+            if self.diagnose_jk:
+                from . import hf
+                td = time.perf_counter()
+                rj, rk = hf.get_jk(mol, dm, 1, self._opt_gpu, out_device=True)
+                e_ref = float(0.5 * torch.sum(dm * (2 * h + rj - 0.5 * rk))) + enuc
+                diag = {'max_dj': float((vj - rj).abs().max()), 'max_dk': float((vk - rk).abs().max()),
+                        'de_vs_fp64_fock': e_tot - e_ref}
+                t_diag += time.perf_counter() - td  # kept out of the timings
             err = f @ dm @ s - s @ dm @ f
             gnorm = float(torch.abs(err).max())
             de = None if e_last is None else e_tot - e_last
             self.history.append({'cycle': cycle, 'e_tot': e_tot, 'delta_e': de, 'grad': gnorm,
-                                 't': time.perf_counter() - t_start, 't_jk': dt_jk,
-                                 'tol': self._tol_now or self.direct_scf_tol})
+                                 't': time.perf_counter() - t_start - t_diag, 't_jk': dt_jk,
+                                 'tol': self._tol_now or self.direct_scf_tol, **diag})
             if self.verbose:
                 print(f'cycle {cycle}: E = {e_tot:.12f}  dE = {de}  |[F,D]| = {gnorm:.2e}', flush=True)
             if de is not None and abs(de) < self.conv_tol and gnorm < self.conv_tol_grad and not loose:
@@ -309,5 +323,6 @@ class RHF:
         self.cycles = len(self.history)
         self.mo_energy, self.mo_coeff = eig(f, s)
         self.dm = dm
-        self.timings.update(guess=t_guess - t_start, scf=t_end - t_guess, jk=t_jk, total=t_end - t_start)
+        self.timings.update(guess=t_guess - t_start, scf=t_end - t_guess - t_diag, jk=t_jk,
+                            total=t_end - t_start - t_diag)
         return self.e_tot
