@@ -226,18 +226,25 @@ def device_mem_used_gb():
     return (total - free) / 1e9
 
 
-def run_scf_diagnosed(mol):
-    """The mixed + learned run again with plain FP64 J/K of every cycle's density built beside it
-    (outside the timings): max|dJ|, max|dK| and the energy difference per iteration."""
+def run_scf_diagnosed(mol, mixed=True):
+    """An SCF again with plain FP64 J/K of every cycle's density built beside it (outside the
+    timings): max|dJ|, max|dK|, the energy difference and the quartets each build computed.
+    mixed: the mixed + learned configuration, else FP64 + SAD."""
     from byteqc_amd.guess.learned import model_path
     from byteqc_amd.scf.rhf import RHF
-    guess = 'learned' if os.path.exists(model_path(mol.basis)) else 'sad'
-    mf = RHF(mol, conv_tol=1e-8, precision_cutoff=1e-7, early_tol=1e-9, early_until=1e-4, diagnose_jk=True)
+    if mixed:
+        guess = 'learned' if os.path.exists(model_path(mol.basis)) else 'sad'
+        mf = RHF(mol, conv_tol=1e-8, precision_cutoff=1e-7, early_tol=1e-9, early_until=1e-4,
+                 diagnose_jk=True)
+    else:
+        guess = 'sad'
+        mf = RHF(mol, conv_tol=1e-8, diagnose_jk=True)
     mf.build()
     mf.kernel(init_guess=guess)
+    keys = ('cycle', 'tol', 't_jk', 'quartets', 'quartets_fp32', 'slots', 'max_dj', 'max_dk',
+            'de_vs_fp64_fock', 'grad')
     return {'guess': guess, 'cycles': mf.cycles, 'e_tot': mf.e_tot,
-            'per_cycle': [{k: h[k] for k in ('cycle', 'tol', 'max_dj', 'max_dk', 'de_vs_fp64_fock', 'grad')}
-                          for h in mf.history]}
+            'per_cycle': [{k: h[k] for k in keys if k in h} for h in mf.history]}
 
 
 def main(argv=None):
@@ -394,6 +401,10 @@ def main(argv=None):
                 f'dE = {r["mixed_learned"]["delta_e_vs_fp64_sad"]:.1e}')
         if args.scf_diagnose:
             r['mixed_learned_per_cycle'] = run_scf_diagnosed(mol)
+            r['fp64_sad_per_cycle'] = run_scf_diagnosed(mol, mixed=False)
+            for name in ('fp64_sad_per_cycle', 'mixed_learned_per_cycle'):
+                log(f'{name}: ' + ' '.join(f'{c["quartets"] / max(c["slots"], 1):.2f}'
+                                           for c in r[name]['per_cycle']) + ' (lanes busy per wave slot)')
             worst = max(r['mixed_learned_per_cycle']['per_cycle'], key=lambda c: c['max_dk'])
             log(f'scf mixed+learned per cycle vs FP64 J/K: worst max|dJ| '
                 f'{max(c["max_dj"] for c in r["mixed_learned_per_cycle"]["per_cycle"]):.1e}, '

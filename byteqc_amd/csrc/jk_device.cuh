@@ -97,6 +97,7 @@ struct JKArgs {
     const double *dm_max;   // device scalar: global max |D| (null: 1)
     long tile_base;         // first tile of this launch
     int kc;                 // ket pairs per tile (register and group kernels)
+    int kc_cap;             // upper bound on kc set by the caller (0: the kernel's default)
     double prim_tol;        // primitive quartets with |prefactor| below this are skipped
     int same;               // bra and ket are the same class: p >= q only
     int nb_bra, nb_ket;     // tiles along bra / ket

@@ -17,6 +17,7 @@
 
 namespace mjk {
 
+constexpr int GRP_KC = 32;  // longest ket chunk per tile, as REG_KC
 
 template <int LA, int LB, int LC, int LD>
 struct GCfg {

@@ -23,7 +23,7 @@ hipError_t launch(JKArgs a, hipStream_t st, long *ntiles_out)
     // ket chunk: long chunks amortise the per-lane bra state, short ones give small classes
     // enough workgroups (about 2048 or more per launch)
     auto pick_kc = [&](long nb_bra, int kmax) {
-        int kc = kmax;
+        int kc = a.kc_cap > 0 && a.kc_cap < kmax ? a.kc_cap : kmax;
         while (kc > 1 && nb_bra * ((a.ket.npairs + kc - 1) / kc) < 2048) kc >>= 1;
         return kc;
     };
@@ -39,7 +39,7 @@ hipError_t launch(JKArgs a, hipStream_t st, long *ntiles_out)
     } else if constexpr (GCfg<LA, LB, LC, LD>::OK) {
         constexpr int W = GCfg<LA, LB, LC, LD>::W;
         a.nb_bra = (a.bra.npairs + W - 1) / W;
-        a.kc = pick_kc(a.nb_bra, 64);
+        a.kc = pick_kc(a.nb_bra, GRP_KC);
         a.nb_ket = (a.ket.npairs + a.kc - 1) / a.kc;
         ntiles = (long)a.nb_bra * a.nb_ket;  // same-class tiles above the triangle exit at once
     } else {

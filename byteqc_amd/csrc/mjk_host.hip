@@ -100,6 +100,12 @@ struct mjk_basis_prod {
     unsigned long long *dm_max = nullptr;  // scalar (double bits)
     unsigned long long *counters = nullptr;  // [0] fp64 quartets, [1] fp32 quartets, [2] quartet slots
     long last_launches = 0;
+    // class pairs of one build run on `lanes` streams: the caller's and lanes - 1 of these
+    int lanes = 4;
+    int kc_cap = 0;  // longest ket chunk per tile (0: each kernel's default)
+    hipStream_t side[MJK_MAX_LANES - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[MJK_MAX_LANES - 1] = {};
+    int n_side = 0;  // side streams created so far
 };
 
 extern "C" {
@@ -286,6 +292,12 @@ oom:
 int mjk_del_basis_prod(mjk_basis_prod **pbp)
 {
     if (!pbp || !*pbp) return MJK_OK;
+    mjk_basis_prod *bp = *pbp;
+    for (int i = 0; i < bp->n_side; ++i) {
+        (void)hipStreamDestroy(bp->side[i]);
+        (void)hipEventDestroy(bp->ev_join[i]);
+    }
+    if (bp->ev_fork) (void)hipEventDestroy(bp->ev_fork);
     (*pbp)->pool.release();
     delete *pbp;
     *pbp = nullptr;
@@ -361,6 +373,7 @@ int run_class_pair(mjk_basis_prod *bp, double *vj, double *vk, const double *dm,
     a.prim_tol = sc.direct_scf_tol * 1e-6;
     a.nbas = bp->nbas; a.nao = bp->nao; a.n_dm = n_dm;
     a.same = cp_ij == cp_kl;
+    a.kc_cap = bp->kc_cap;
     const double pc = fp32_pays(bra->l_bra, bra->l_ket, ket->l_bra, ket->l_ket) ? sc.precision_cutoff : 0.0;
     long nt = 0;
     // FP64 part: bound >= precision_cutoff (everything when the cutoff is 0)
@@ -408,9 +421,47 @@ int ensure_dm_cond(mjk_basis_prod *bp)
     return MJK_OK;
 }
 
+// Side streams (default priority, not synchronising with the null stream) and their
+// events, created on the cache's device the first time a build asks for them.
+int ensure_lanes(mjk_basis_prod *bp, int lanes)
+{
+    while (bp->n_side < lanes - 1) {
+        const int i = bp->n_side;
+        hipError_t e = hipStreamCreateWithFlags(&bp->side[i], hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(e, "side stream");
+        e = hipEventCreateWithFlags(&bp->ev_join[i], hipEventDisableTiming);
+        if (e != hipSuccess) {
+            (void)hipStreamDestroy(bp->side[i]);
+            return hip_fail(e, "side stream event");
+        }
+        bp->n_side = i + 1;
+    }
+    if (lanes > 1 && !bp->ev_fork) {
+        hipError_t e = hipEventCreateWithFlags(&bp->ev_fork, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(e, "fork event");
+    }
+    return MJK_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mjk_set_streams(mjk_basis_prod *bp, int lanes)
+{
+    if (!bp) return fail(MJK_ERR_ARG, "mjk_set_streams: null basis-product cache");
+    if (lanes < 1 || lanes > MJK_MAX_LANES) return fail(MJK_ERR_ARG, "mjk_set_streams: streams out of range");
+    bp->lanes = lanes;
+    return MJK_OK;
+}
+
+int mjk_set_ket_chunk(mjk_basis_prod *bp, int cap)
+{
+    if (!bp) return fail(MJK_ERR_ARG, "mjk_set_ket_chunk: null basis-product cache");
+    if (cap < 0 || cap > 4096) return fail(MJK_ERR_ARG, "mjk_set_ket_chunk: chunk out of range");
+    bp->kc_cap = cap;
+    return MJK_OK;
+}
 
 int mjk_build_jk(mjk_basis_prod *bp, double *vj, double *vk, const double *dm, int nao, int n_dm,
                  int cp_ij, int cp_kl, const mjk_screen *screen, void *stream)
@@ -459,10 +510,38 @@ int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *d
             order.push_back({-w, {i, j}});
         }
     std::sort(order.begin(), order.end());
-    for (auto &o : order) {
-        rc = run_class_pair(bp, vj, vk, dm, n_dm, o.second.first, o.second.second, *screen, dens, st);
-        if (rc) return rc;
+    // The class pairs are spread over several streams.  A launch lasts as long as its
+    // longest chain of ket iterations; when density screening leaves few quartets (the
+    // incremental builds of an SCF) that tail, not the work, sets its duration, and
+    // concurrent launches overlap the tails.  Each class pair goes to the stream with the
+    // least modelled work so far; the caller's stream waits for all of them.
+    const int lanes = bp->lanes;
+    if ((rc = ensure_lanes(bp, lanes)) != MJK_OK) return rc;
+    hipStream_t ls[MJK_MAX_LANES];
+    double load[MJK_MAX_LANES];
+    ls[0] = st;
+    load[0] = 0.0;
+    for (int k = 1; k < lanes; ++k) {
+        ls[k] = bp->side[k - 1];
+        load[k] = 0.0;
     }
+    if (lanes > 1) {
+        if ((e = hipEventRecord(bp->ev_fork, st)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: fork");
+        for (int k = 1; k < lanes; ++k)
+            if ((e = hipStreamWaitEvent(ls[k], bp->ev_fork, 0)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: fork");
+    }
+    for (auto &o : order) {
+        const int k = (int)(std::min_element(load, load + lanes) - load);
+        load[k] -= o.first;
+        rc = run_class_pair(bp, vj, vk, dm, n_dm, o.second.first, o.second.second, *screen, dens, ls[k]);
+        if (rc) break;
+    }
+    // join every lane, also after a failed launch, so no work is left behind the caller's stream
+    for (int k = 1; k < lanes; ++k) {
+        if ((e = hipEventRecord(bp->ev_join[k - 1], ls[k])) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: join");
+        if ((e = hipStreamWaitEvent(st, bp->ev_join[k - 1], 0)) != hipSuccess) return hip_fail(e, "mjk_build_jk_all: join");
+    }
+    if (rc) return rc;
     if (vj && (e = launch_symmetrize(vj, n_dm, nao, 2.0, st)) != hipSuccess) return hip_fail(e, "symmetrize vj");
     if (vk && (e = launch_symmetrize(vk, n_dm, nao, 1.0, st)) != hipSuccess) return hip_fail(e, "symmetrize vk");
     return MJK_OK;

@@ -212,6 +212,16 @@ class _VHFOpt:
     def npairs(self):
         return int(self.bas_pairs_locs[-1])
 
+    def set_streams(self, lanes):
+        """Streams a build spreads its class pairs over (1 = the caller's stream only)."""
+        mjk.check(mjk.load().mjk_set_streams(self.bp, int(lanes)), 'mjk_set_streams')
+        return self
+
+    def set_ket_chunk(self, cap):
+        """Longest ket chunk per workgroup of the register and group kernels (0: default)."""
+        mjk.check(mjk.load().mjk_set_ket_chunk(self.bp, int(cap)), 'mjk_set_ket_chunk')
+        return self
+
     def stats(self):
         out = np.zeros(4)
         mjk.check(mjk.load().mjk_last_stats(self.bp, mjk.ptr(out)), 'mjk_last_stats')

@@ -36,15 +36,23 @@ def int1e(vhfopt, nthreads=0):
     return c.T @ s @ c, c.T @ t @ c, c.T @ v @ c
 
 
-def eig(h, s, threshold=1e-15):
-    """Generalised eigenproblem h c = s c e by canonical orthogonalisation (hf.py:253-265).
-
-    h, s: torch tensors on the GPU.  Eigenvectors of s with eigenvalue below
-    `threshold` are dropped."""
+def orthogonalizer(s, threshold=1e-15):
+    """Canonical orthogonaliser t (t^T s t = 1) of a GPU overlap matrix; eigenvectors of s
+    with eigenvalue below `threshold` are dropped."""
     torch = _torch()
     seig, t = torch.linalg.eigh(s)
     keep = seig >= threshold
-    t = t[:, keep] / torch.sqrt(seig[keep])
+    return t[:, keep] / torch.sqrt(seig[keep])
+
+
+def eig(h, s, threshold=1e-15, t=None):
+    """Generalised eigenproblem h c = s c e by canonical orthogonalisation (hf.py:253-265).
+
+    h, s: torch tensors on the GPU.  t: s's orthogonaliser when the caller keeps it (an SCF
+    solves with one overlap matrix every cycle), else computed here."""
+    torch = _torch()
+    if t is None:
+        t = orthogonalizer(s, threshold)
     heff = t.T @ h @ t
     e, c = torch.linalg.eigh(heff)
     return e, t @ c
@@ -261,6 +269,7 @@ class RHF:
         enuc = self.energy_nuc()
         nocc = mol.nelectron // 2
         diis = DIIS(self.diis_space)
+        orth = orthogonalizer(s)  # s is fixed through the SCF: one diagonalisation of it
         vj = vk = None
         dm_last = None
         e_last = None
@@ -294,10 +303,15 @@ class RHF:
             if self.diagnose_jk:
                 from . import hf
                 td = time.perf_counter()
+                if self.jk is None:
+                    # work of the timed build: computed quartets and the wave slots they occupied
+                    st = self._opt_gpu.stats()
+                    diag = {'quartets': st['quartets_fp64'] + st['quartets_fp32'],
+                            'quartets_fp32': st['quartets_fp32'], 'slots': st['slots']}
                 rj, rk = hf.get_jk(mol, dm, 1, self._opt_gpu, out_device=True)
                 e_ref = float(0.5 * torch.sum(dm * (2 * h + rj - 0.5 * rk))) + enuc
-                diag = {'max_dj': float((vj - rj).abs().max()), 'max_dk': float((vk - rk).abs().max()),
-                        'de_vs_fp64_fock': e_tot - e_ref}
+                diag.update(max_dj=float((vj - rj).abs().max()), max_dk=float((vk - rk).abs().max()),
+                            de_vs_fp64_fock=e_tot - e_ref)
                 t_diag += time.perf_counter() - td  # kept out of the timings
             err = f @ dm @ s - s @ dm @ f
             gnorm = float(torch.abs(err).max())
@@ -312,7 +326,7 @@ class RHF:
                 e_last = e_tot
                 break
             fd = diis.update(f, err)
-            mo_e, mo_c = eig(fd, s)
+            mo_e, mo_c = eig(fd, s, t=orth)
             dm_last = dm
             co = mo_c[:, :nocc]
             dm = 2.0 * co @ co.T
@@ -321,7 +335,7 @@ class RHF:
         t_end = time.perf_counter()
         self.e_tot = e_last
         self.cycles = len(self.history)
-        self.mo_energy, self.mo_coeff = eig(f, s)
+        self.mo_energy, self.mo_coeff = eig(f, s, t=orth)
         self.dm = dm
         self.timings.update(guess=t_guess - t_start, scf=t_end - t_guess - t_diag, jk=t_jk,
                             total=t_end - t_start - t_diag)

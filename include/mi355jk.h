@@ -96,6 +96,18 @@ int mjk_build_jk(mjk_basis_prod *bp, double *vj, double *vk, const double *dm,
 int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *dm,
                      int nao, int n_dm, const mjk_screen *screen, void *stream);
 
+/* Streams one mjk_build_jk_all spreads its class pairs over: the caller's stream plus
+ * lanes - 1 side streams of the cache (default priority), joined back into the caller's
+ * stream before the call's last kernels.  1 runs everything on the caller's stream;
+ * default 4.  No counterpart in the reference, whose build launches on one stream. */
+#define MJK_MAX_LANES 8
+int mjk_set_streams(mjk_basis_prod *bp, int lanes);
+
+/* Longest chunk of ket pairs one workgroup of the register and group kernels walks
+ * (0: the kernels' default, 32).  Shorter chunks mean more, shorter tiles.
+ * No counterpart in the reference. */
+int mjk_set_ket_chunk(mjk_basis_prod *bp, int cap);
+
 /* Counters of the last mjk_build_jk_all on this cache (host values, synchronises):
  * out[0] quartets computed in FP64, out[1] in FP32, out[2] kernel launches, out[3] quartet
  * slots the register and group kernels executed (lanes or lane groups times ket
