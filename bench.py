@@ -201,7 +201,10 @@ def run_scf(mol, precision_cutoff=0.0, guess='sad', early_tol=None, early_until=
     (guess included; the SAD atomic densities and the learned guess's isolated-fragment density
     are cached beforehand, as for a warm code)."""
     from byteqc_amd.scf.rhf import RHF, init_guess_by_atom
+    import torch
     init_guess_by_atom(mol)  # warm the atomic-density cache
+    a = torch.rand(mol.nao, mol.nao, dtype=torch.float64, device='cuda')
+    torch.linalg.eigh(a + a.T)  # the eigensolver's first call of a size sets up its workspace
     if guess.startswith('learned'):
         from byteqc_amd.guess.fragments import fragment_density
         fragment_density(mol)  # warm the isolated-fragment density cache, as for SAD's atoms

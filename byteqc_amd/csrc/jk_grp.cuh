@@ -17,7 +17,9 @@
 
 namespace mjk {
 
-constexpr int GRP_KC = 32;  // longest ket chunk per tile, as REG_KC
+// longest ket chunk per tile: shorter chunks give more tiles and shorter per-tile chains of
+// ket iterations, which is what bounds a sparse (density-screened) launch of the large classes
+constexpr int GRP_KC = 32;
 
 template <int LA, int LB, int LC, int LD>
 struct GCfg {

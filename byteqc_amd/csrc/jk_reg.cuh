@@ -38,10 +38,7 @@ constexpr int reg_min_blocks(int la, int lb, int lc, int ld)
         return 1;
     }
 }
-// longest ket chunk per tile: shorter chunks give more tiles and shorter per-tile chains of
-// ket iterations, which is what bounds a sparse (density-screened) launch; 32 measured best
-// for dense and incremental builds alike on (H2O)64/def2-TZVP
-constexpr int REG_KC = 32;
+constexpr int REG_KC = 128;   // longest ket chunk per tile
 
 // Tiles of a same-class launch with ket chunks of kc pairs (kc divides REG_BRA): bra block
 // b needs ket chunks 0 .. min(nb_ket, (b+1) r) - 1, r = REG_BRA / kc.  Returns the number of

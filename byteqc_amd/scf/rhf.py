@@ -166,7 +166,7 @@ class RHF:
 
     def __init__(self, mol, conv_tol=1e-8, conv_tol_grad=None, max_cycle=100, diis_space=8,
                  direct_scf_tol=1e-13, precision_cutoff=0.0, incremental=True,
-                 rebuild_every=8, verbose=0, jk=None, early_tol=None, early_until=1e-3,
+                 rebuild_every=None, verbose=0, jk=None, early_tol=None, early_until=1e-3,
                  diagnose_jk=False):
         self.mol = mol
         self.conv_tol = conv_tol
@@ -176,7 +176,10 @@ class RHF:
         self.direct_scf_tol = direct_scf_tol
         self.precision_cutoff = precision_cutoff
         self.incremental = incremental
-        self.rebuild_every = rebuild_every
+        # incremental Fock builds on dm - dm_last; rebuild_every=n forces a full build after n
+        # incremental ones (None: never, as PySCF's direct SCF, which the reference's RHF
+        # inherits: cuobc/scf/hf.py:272-274)
+        self.rebuild_every = np.inf if rebuild_every is None else rebuild_every
         self.verbose = verbose
         self.jk = jk
         # looser quartet screening while the orbital gradient max|[F, D]| exceeds early_until;
@@ -276,6 +279,7 @@ class RHF:
         self.history = []
         self.converged = False
         since_full = 0
+        force_full = False
         t_jk = 0.0
         t_diag = 0.0
         loose = self.early_tol is not None and self.early_tol > self.direct_scf_tol
@@ -285,8 +289,8 @@ class RHF:
             if loose and self.history and self.history[-1]['grad'] < self.early_until:
                 loose = False
                 self._tol_now = None
-                since_full = self.rebuild_every  # full rebuild at the tight tolerance
-            if self.incremental and dm_last is not None and since_full < self.rebuild_every:
+                force_full = True  # full rebuild at the tight tolerance
+            if self.incremental and dm_last is not None and since_full < self.rebuild_every and not force_full:
                 dj, dk = self.get_jk(mol, dm - dm_last)
                 vj = vj + dj
                 vk = vk + dk
@@ -294,6 +298,7 @@ class RHF:
             else:
                 vj, vk = self.get_jk(mol, dm)
                 since_full = 0
+                force_full = False
             torch.cuda.synchronize()
             dt_jk = time.perf_counter() - tj
             t_jk += dt_jk

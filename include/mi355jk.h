@@ -104,7 +104,7 @@ int mjk_build_jk_all(mjk_basis_prod *bp, double *vj, double *vk, const double *d
 int mjk_set_streams(mjk_basis_prod *bp, int lanes);
 
 /* Longest chunk of ket pairs one workgroup of the register and group kernels walks
- * (0: the kernels' default, 32).  Shorter chunks mean more, shorter tiles.
+ * (0: the kernels' defaults, 128 and 32).  Shorter chunks mean more, shorter tiles.
  * No counterpart in the reference. */
 int mjk_set_ket_chunk(mjk_basis_prod *bp, int cap);
 
