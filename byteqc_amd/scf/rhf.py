@@ -64,6 +64,7 @@ class DIIS:
     def __init__(self, space=8):
         self.space = space
         self.focks, self.errs = [], []
+        self.gram = None  # error overlaps <e_i, e_j>, one new row per update
 
     def update(self, f, err):
         torch = _torch()
@@ -72,15 +73,22 @@ class DIIS:
         if len(self.focks) > self.space:
             self.focks.pop(0)
             self.errs.pop(0)
+            self.gram = self.gram[1:, 1:]
         n = len(self.focks)
+        row = torch.stack([torch.sum(x * err) for x in self.errs])
+        g = torch.empty((n, n), dtype=torch.float64, device=f.device)
+        if n > 1:
+            g[:n - 1, :n - 1] = self.gram
+        g[n - 1, :] = row
+        g[:, n - 1] = row
+        self.gram = g
         if n < 2:
             return f
         b = torch.empty((n + 1, n + 1), dtype=torch.float64, device=f.device)
         b[-1, :] = -1
         b[:, -1] = -1
         b[-1, -1] = 0
-        e = torch.stack([x.reshape(-1) for x in self.errs])
-        b[:n, :n] = e @ e.T
+        b[:n, :n] = g
         rhs = torch.zeros(n + 1, dtype=torch.float64, device=f.device)
         rhs[-1] = -1
         try:

@@ -48,6 +48,8 @@ def test_version_and_error_paths():
     sc = mjk.Screen(1e-13, 0.0, 1)
     rc = lib.mjk_build_jk_all(None, None, None, None, 10, 1, ctypes.byref(sc), None)
     assert rc == mjk.MJK_ERR_ARG
+    assert lib.mjk_set_streams(None, 4) == mjk.MJK_ERR_ARG
+    assert lib.mjk_set_ket_chunk(None, 32) == mjk.MJK_ERR_ARG
 
 
 def test_check_raises():

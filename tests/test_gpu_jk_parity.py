@@ -61,3 +61,29 @@ def test_j_only_k_only():
     assert none_k is None and none_j is None
     assert np.abs(j_only - vj).max() < 1e-13
     assert np.abs(k_only - vk).max() < 1e-13
+
+
+@pytest.fixture(scope='module')
+def tzvp4():
+    mol = water_cluster(4, 9.0, seed=11, basis='def2-tzvp')
+    dm = _rand_dm(mol.nao, 5)
+    return mol, dm, oracle.get_jk(mol, dm, cutoff=None)
+
+
+@pytest.mark.parametrize('lanes', [1, 2, 4])
+def test_streams_and_ket_chunks_agree_with_oracle(tzvp4, lanes):
+    """Class pairs spread over 1, 2 or 4 streams, with default and short ket chunks, give the
+    oracle's J/K (1e-10), also when the builder is reused on the same cache."""
+    mol, dm, (rj, rk) = tzvp4
+    print(f'case: (H2O)4 def2-tzvp nao={mol.nao}, streams {lanes}', flush=True)
+    opt = hf._VHFOpt(mol).build()
+    try:
+        opt.set_streams(lanes)
+        for cap in (0, 4):
+            opt.set_ket_chunk(cap)
+            for _ in range(2):
+                vj, vk = hf.get_jk(mol, dm, vhfopt=opt, prescreen=False)
+                assert np.abs(vj - rj).max() < 1e-10
+                assert np.abs(vk - rk).max() < 1e-10
+    finally:
+        opt.clear()

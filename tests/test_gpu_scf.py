@@ -60,3 +60,22 @@ def test_incremental_matches_full():
     e1 = RHF(mol, conv_tol=1e-10, incremental=True).kernel()
     e2 = RHF(mol, conv_tol=1e-10, incremental=False).kernel()
     assert abs(e1 - e2) < 1e-9
+
+
+def test_diis_gram_kept_across_updates():
+    """DIIS keeps <e_i, e_j> row by row; after the subspace is full and old vectors drop out,
+    it equals the Gram matrix of the vectors it holds, and the extrapolation weights sum to 1."""
+    import torch
+    from byteqc_amd.scf.rhf import DIIS
+    g = torch.Generator(device='cuda').manual_seed(3)
+    d = DIIS(space=4)
+    for _ in range(7):
+        f = torch.rand((12, 12), dtype=torch.float64, device='cuda', generator=g)
+        e = torch.rand((12, 12), dtype=torch.float64, device='cuda', generator=g) - 0.5
+        out = d.update(f, e)
+    stack = torch.stack([x.reshape(-1) for x in d.errs])
+    assert len(d.errs) == 4
+    assert torch.allclose(d.gram, stack @ stack.T, rtol=1e-13, atol=1e-13)
+    # out is an affine combination of the stored Fock matrices
+    coef = torch.linalg.lstsq(torch.stack([x.reshape(-1) for x in d.focks]).T, out.reshape(-1, 1)).solution
+    assert abs(float(coef.sum()) - 1.0) < 1e-8
