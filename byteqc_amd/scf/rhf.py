@@ -236,6 +236,8 @@ class RHF:
         # an eigendecomposition; the eigensolver remains the fallback and gives the final
         # orbitals
         self.purification = purification
+        # path of a checkpoint (PySCF's chkfile keys in .npz) written after kernel()
+        self.chkfile = None
         self._opt_gpu = None
         self.converged = False
         self.e_tot = None
@@ -390,7 +392,12 @@ class RHF:
         self.e_tot = e_last
         self.cycles = len(self.history)
         self.mo_energy, self.mo_coeff = eig(f, s, t=orth)
+        self.mo_occ = np.zeros(self.mo_energy.shape[0])
+        self.mo_occ[:nocc] = 2.0
         self.dm = dm
         self.timings.update(guess=t_guess - t_start, scf=t_end - t_guess - t_diag, jk=t_jk,
                             total=t_end - t_start - t_diag)
+        if self.chkfile:
+            from . import chkfile
+            chkfile.dump_scf(self.chkfile, mol, self.e_tot, self.mo_energy, self.mo_coeff, self.mo_occ)
         return self.e_tot

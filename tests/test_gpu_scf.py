@@ -79,3 +79,19 @@ def test_diis_gram_kept_across_updates():
     # out is an affine combination of the stored Fock matrices
     coef = torch.linalg.lstsq(torch.stack([x.reshape(-1) for x in d.focks]).T, out.reshape(-1, 1)).solution
     assert abs(float(coef.sum()) - 1.0) < 1e-8
+
+
+def test_rhf_writes_a_checkpoint(tmp_path):
+    """RHF(chkfile=...) leaves PySCF's scf keys: the energy, and orbitals that reproduce the
+    converged density from the occupations."""
+    from byteqc_amd.scf import chkfile
+    mol = M('O 0 0 0; H 0.757 0 0.587; H -0.757 0 0.587', basis='cc-pvdz')
+    mf = RHF(mol, conv_tol=1e-9)
+    mf.chkfile = str(tmp_path / 'h2o.npz')
+    e = mf.kernel()
+    mol2, scf = chkfile.load_scf(mf.chkfile)
+    assert scf['e_tot'] == e and mol2.nao == mol.nao
+    c, occ = scf['mo_coeff'], scf['mo_occ']
+    assert occ.sum() == mol.nelectron
+    dm = (c * occ) @ c.T
+    assert np.abs(dm - mf.dm.cpu().numpy()).max() < 1e-6
