@@ -82,8 +82,9 @@ def test_diis_gram_kept_across_updates():
 
 
 def test_rhf_writes_a_checkpoint(tmp_path):
-    """RHF(chkfile=...) leaves PySCF's scf keys: the energy, and orbitals that reproduce the
-    converged density from the occupations."""
+    """RHF(chkfile=...) leaves PySCF's scf keys: the energy, and S-orthonormal orbitals of the
+    last Fock matrix, whose density is one step beyond the one that built it (the difference
+    is the convergence residual)."""
     from byteqc_amd.scf import chkfile
     mol = M('O 0 0 0; H 0.757 0 0.587; H -0.757 0 0.587', basis='cc-pvdz')
     mf = RHF(mol, conv_tol=1e-9)
@@ -94,4 +95,5 @@ def test_rhf_writes_a_checkpoint(tmp_path):
     c, occ = scf['mo_coeff'], scf['mo_occ']
     assert occ.sum() == mol.nelectron
     dm = (c * occ) @ c.T
-    assert np.abs(dm - mf.dm.cpu().numpy()).max() < 1e-6
+    assert np.abs(c.T @ mf._s @ c - np.eye(c.shape[1])).max() < 1e-10
+    assert np.abs(dm - mf.dm.cpu().numpy()).max() < 1e-4
