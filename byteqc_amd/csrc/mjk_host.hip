@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -27,7 +28,7 @@ int fail(int code, const std::string &msg)
 
 int hip_fail(hipError_t e, const char *where)
 {
-    return fail(MJK_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+    return fail(e == hipErrorOutOfMemory ? MJK_ERR_OOM : MJK_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
 }  // namespace
@@ -49,6 +50,9 @@ constexpr int PTR_COORD = 1;
 double fac_sp_host(int l) { return l == 0 ? 0.282094791773878143 : (l == 1 ? 0.488602511902919921 : 1.0); }
 
 // A list of device allocations released together.
+// Allocations left before an injected failure (mjk_fail_alloc_after; -1: none injected).
+std::atomic<long> g_alloc_budget{-1};
+
 struct DevPool {
     std::vector<void *> ptrs;
     template <typename T>
@@ -56,6 +60,10 @@ struct DevPool {
     {
         *p = nullptr;
         if (n == 0) n = 1;
+        for (long b = g_alloc_budget.load(); b >= 0;) {
+            if (b == 0) return hipErrorOutOfMemory;
+            if (g_alloc_budget.compare_exchange_weak(b, b - 1)) break;
+        }
         hipError_t e = hipMalloc((void **)p, n * sizeof(T));
         if (e == hipSuccess) ptrs.push_back(*p);
         return e;
@@ -446,6 +454,12 @@ int ensure_lanes(mjk_basis_prod *bp, int lanes)
 }  // namespace
 
 extern "C" {
+
+int mjk_fail_alloc_after(int n)
+{
+    g_alloc_budget.store(n < 0 ? -1 : (long)n);
+    return MJK_OK;
+}
 
 int mjk_set_streams(mjk_basis_prod *bp, int lanes)
 {

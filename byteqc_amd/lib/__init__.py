@@ -30,7 +30,7 @@ class Screen(ctypes.Structure):
 
 EXPORTS = ('mjk_version', 'mjk_last_error', 'mjk_q_cond', 'mjk_init_basis_prod',
            'mjk_del_basis_prod', 'mjk_nao', 'mjk_build_jk', 'mjk_build_jk_all',
-           'mjk_last_stats', 'mjk_set_streams', 'mjk_set_ket_chunk', 'mjk_int1e', 'mjk_gemm_bf16', 'mjk_bias_act', 'mjk_act_backward',
+           'mjk_last_stats', 'mjk_set_streams', 'mjk_set_ket_chunk', 'mjk_fail_alloc_after', 'mjk_int1e', 'mjk_gemm_bf16', 'mjk_bias_act', 'mjk_act_backward',
            'mjk_mse_grad', 'mjk_adam', 'mjk_to_bf16', 'mjk_batch_create', 'mjk_batch_destroy',
            'mjk_batch_build', 'mjk_batch_num_class_pairs', 'mjk_batch_tiles', 'mjk_batch_set_kc',
            'mjk_batch_time_class_pair', 'mjk_batch_stats', 'mjk_fill_int2e')
@@ -70,6 +70,7 @@ def load():
     lib.mjk_last_stats.argtypes = [p, p]
     lib.mjk_set_streams.argtypes = [p, ctypes.c_int]
     lib.mjk_set_ket_chunk.argtypes = [p, ctypes.c_int]
+    lib.mjk_fail_alloc_after.argtypes = [ctypes.c_int]
     lib.mjk_int1e.argtypes = [p, i, p, i, p, p, p, p, i]
     f, sz = ctypes.c_float, ctypes.c_size_t
     lib.mjk_gemm_bf16.argtypes = [i, i, i, i, p, i, p, i, p, i, f, p]

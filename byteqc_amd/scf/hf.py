@@ -298,6 +298,10 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None, 
         raise NotImplementedError('JK-builder only supports hermitian density matrix')
     if omega is not None:
         raise NotImplementedError('range-separated J/K (omega) is not supported')
+    is_tensor = hasattr(dm, 'data_ptr') and not isinstance(dm, np.ndarray)
+    # a NaN or Inf in D would pass the screening (max |D| ignores NaN) and spread through J/K
+    if not is_tensor and not np.isfinite(np.asarray(dm)).all():
+        raise ValueError('get_jk: dm contains NaN or Inf')
     torch = _torch()
     if vhfopt is None:
         vhfopt = _VHFOpt(mol, 'int2e').build(group_size=group_size, gpus=gpus)
@@ -307,9 +311,10 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None, 
         return None, None
     c, ct = vhfopt.coeff_dev()
     nao0 = c.shape[1]
-    is_tensor = hasattr(dm, 'data_ptr') and not isinstance(dm, np.ndarray)
     dms = (dm if is_tensor else torch.as_tensor(np.asarray(dm), dtype=torch.float64))
     dms = dms.to(device='cuda', dtype=torch.float64)
+    if is_tensor and not bool(torch.isfinite(dms).all()):
+        raise ValueError('get_jk: dm contains NaN or Inf')
     shape = dms.shape
     dms = dms.reshape(-1, nao0, nao0)
     dmc = torch.matmul(torch.matmul(c, dms), ct)

@@ -108,6 +108,12 @@ int mjk_set_streams(mjk_basis_prod *bp, int lanes);
  * No counterpart in the reference. */
 int mjk_set_ket_chunk(mjk_basis_prod *bp, int cap);
 
+/* Test hook for failure handling: the next n device allocations of the library's caches
+ * succeed and every later one fails with MJK_ERR_OOM (no device memory is touched for it),
+ * until called with n < 0.  Process-wide.  No counterpart in the reference, whose allocation
+ * helpers exit() on a device error (cuobc/lib/gint/cuda_alloc.cuh:22-31). */
+int mjk_fail_alloc_after(int n);
+
 /* Counters of the last mjk_build_jk_all on this cache (host values, synchronises):
  * out[0] quartets computed in FP64, out[1] in FP32, out[2] kernel launches, out[3] quartet
  * slots the register and group kernels executed (lanes or lane groups times ket
