@@ -4,7 +4,9 @@
  * int status (MJK_OK on success) and never exit the process.  The text of the last
  * error on the calling thread is returned by mjk_last_error().  Device pointers are
  * HIP device allocations on the current device; `stream` is a hipStream_t passed as
- * void* (NULL = the null stream).  Launches are asynchronous on that stream.
+ * void* (NULL = the null stream).  Launches are asynchronous on that stream.  A cache
+ * (mjk_basis_prod) is used by one host thread at a time: its builds share the cache's
+ * side streams, counters and density-maxima workspace; separate caches are independent.
  *
  * The entry points replace ByteQC's ctypes interface to libgint/libgvhf:
  *   mjk_q_cond              <- PySCF VHFOpt.get_q_cond (CVHFsetnr_direct_scf), used at
